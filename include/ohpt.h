@@ -1,0 +1,240 @@
+/*
+ * ohpt.h — C-ABI of the MI355X path-tracing core.
+ *
+ * Plain C types only: pointers, sizes, PODs. No exceptions cross this
+ * boundary. Every function returns OHPT_OK (0) or a negative error code and
+ * leaves a message for ohpt_last_error() on the calling thread.
+ *
+ * One context owns one HIP stream on one device. Contexts are not
+ * thread-safe; use one per host thread (the reference renderer is
+ * single-threaded: ohao/gpu/vulkan/renderer.cpp:729-733).
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   VulkanRenderer façade        ohao/gpu/vulkan/renderer.hpp:114-292
+ *   IRTRendererProfile plugin    ohao/render/rt/rt_profile_renderer.hpp:7-86
+ *   PathTracer::render           ohao/render/rt/path_tracer_render.cpp:38-1281
+ *   scene→RT upload              ohao/gpu/vulkan/rt_build.cpp:825-906,
+ *                                ohao/gpu/vulkan/light_upload.cpp:153-501
+ */
+#ifndef OHPT_H
+#define OHPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OHPT_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+    OHPT_OK = 0,
+    OHPT_ERR_INVALID_ARG = -1,   /* null handle/pointer, bad size          */
+    OHPT_ERR_HIP = -2,           /* a HIP runtime call failed              */
+    OHPT_ERR_NO_SCENE = -3,      /* render before a scene was committed    */
+    OHPT_ERR_OUT_OF_MEMORY = -4,
+    OHPT_ERR_UNSUPPORTED = -5,
+    OHPT_ERR_STATE = -6          /* call out of order (e.g. adjoint without forward state) */
+};
+
+/* ---- samplers (ohao/render/rt/sampler_types.hpp; shaders/includes/rt/sampler_api.glsl:16-41) */
+enum {
+    OHPT_SAMPLER_PCG32 = 0,      /* O'Neill PCG32, stream = pixel, state seeded from (seed, sample, vertex) */
+    OHPT_SAMPLER_SOBOL_OWEN = 1  /* padded-4D Sobol + Burley-2020 Owen scramble (reference default) */
+};
+
+/* ---- render flags ------------------------------------------------------ */
+enum {
+    OHPT_FLAG_AOVS = 1u << 0,          /* write first-hit albedo/normal/depth AOVs (PT_FLAG_ENABLE_AOVS) */
+    OHPT_FLAG_FIREFLY_CLAMP = 1u << 2, /* luminance clamp per sample (PT_FLAG_ENABLE_FIREFLY_CLAMP) */
+    OHPT_FLAG_NO_ENV_NEE = 1u << 8,    /* disable env importance sampling (BSDF sampling only) */
+    OHPT_FLAG_NO_LIGHT_NEE = 1u << 9   /* disable light/emitter NEE (BSDF sampling only) */
+};
+
+/* ---- light types (ohao/scene/component/light_component.hpp:10-15) ------- */
+enum {
+    OHPT_LIGHT_SPHERE = 0,
+    OHPT_LIGHT_DIRECTIONAL = 1,
+    OHPT_LIGHT_SPOT = 2,
+    OHPT_LIGHT_AREA_RECT = 3
+};
+
+/*
+ * Light record: byte-for-byte the reference GPULight (80 B, std430),
+ * ohao/render/rt/gpu_light.hpp:10-16, filled as in
+ * ohao/gpu/vulkan/light_upload.cpp:157-181.
+ */
+typedef struct ohpt_light {
+    float position_type[4];        /* xyz position, w = type                    */
+    float color_intensity[4];      /* rgb color, w = intensity                  */
+    float dir_param[4];            /* xyz direction, w = radius / inner angle   */
+    float extra[4];                /* area: xyz edge1; spot: w = outer angle    */
+    float extra2[4];               /* area: xyz edge2, w = area                 */
+} ohpt_light;
+
+/*
+ * Material record (64 B). Mirrors the reference Material fields that reach
+ * the ray-traced path (ohao/gpu/vulkan/material.hpp:11-41, packed in
+ * ohao/gpu/vulkan/rt_build.cpp:236-257). `emissive` is uploaded here although
+ * the reference drops it on this path; `specular_f0` is the dielectric F0
+ * the reference hard-codes to 0.04 (0 gives a pure Lambert surface).
+ * Texture ids index textures uploaded with ohpt_upload_texture; -1 = none.
+ */
+typedef struct ohpt_material {
+    float base_color[3];
+    float roughness;
+    float metallic;
+    float specular_f0;
+    float emissive[3];
+    int32_t albedo_tex;            /* RGB texture, multiplies base_color        */
+    int32_t roughness_tex;         /* channel 0 multiplies roughness            */
+    int32_t metallic_tex;          /* channel 0 multiplies metallic             */
+    int32_t reserved[4];
+} ohpt_material;
+
+/*
+ * Render settings (subset of RTRenderSettings, ohao/render/rt/rt_settings.hpp:17-35,
+ * that affects the offline integrator).
+ */
+typedef struct ohpt_settings {
+    uint32_t max_bounces;          /* indirect bounces; extend rays per path = max_bounces + 1 */
+    uint32_t rr_start;             /* first bounce index that applies Russian roulette */
+    uint32_t sampler;              /* OHPT_SAMPLER_*                            */
+    uint32_t flags;                /* OHPT_FLAG_*                               */
+    float firefly_clamp;           /* luminance clamp when OHPT_FLAG_FIREFLY_CLAMP */
+    float ray_eps;                 /* relative origin offset for secondary rays (0 = default 1e-4) */
+    uint32_t reserved[2];
+} ohpt_settings;
+
+/* Pinhole camera in the reference's convention
+ * (shaders/rt/pt_raygen_offline.rgen:144-162, render_dispatch.cpp:161-163):
+ * dir = normalize(fwd + right*ndc.x*tan_y*aspect - up*ndc.y*tan_y). */
+typedef struct ohpt_camera {
+    float position[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float tan_half_fov_y;
+} ohpt_camera;
+
+/* GPU timing of the last ohpt_render / ohpt_adjoint call (HIP events). */
+typedef struct ohpt_stats {
+    double ms_total;               /* generate -> accumulate                    */
+    double ms_extend;              /* closest-hit traversal kernels             */
+    double ms_shade;
+    double ms_connect;             /* shadow-ray kernels                        */
+    double ms_build;               /* last LBVH build                           */
+    uint64_t rays_extend;          /* closest-hit rays traced                   */
+    uint64_t rays_shadow;          /* shadow rays traced                        */
+    uint64_t paths;
+    uint64_t node_visits;          /* extend-kernel BVH node visits (OHPT_RENDER_PROFILE only) */
+    uint64_t tri_tests;            /* extend-kernel ray/triangle tests (OHPT_RENDER_PROFILE only) */
+    uint32_t extend_launches;      /* extend-kernel launches in the last call   */
+    uint32_t reserved;
+} ohpt_stats;
+
+typedef struct ohpt_context ohpt_context;
+
+/* ---- lifecycle -------------------------------------------------------------
+ * replaces VulkanRenderer(W,H) + initialize() (renderer.hpp:116-121),
+ * resize (renderer.hpp:125), shutdown (renderer.hpp:122). */
+int ohpt_create(ohpt_context** out, int device, uint32_t width, uint32_t height);
+int ohpt_destroy(ohpt_context* ctx);
+int ohpt_resize(ohpt_context* ctx, uint32_t width, uint32_t height);
+const char* ohpt_last_error(void);
+int ohpt_abi_version(void);
+
+/* ---- scene upload ------------------------------------------------------------
+ * replaces updateSceneBuffers/buildBLASTLAS (scene_upload.cpp:21-190,
+ * rt_build.cpp:825-906). Meshes are transformed to world space with the
+ * column-major 4x4 `world` matrix (glm convention) and flattened into one
+ * LBVH built on the GPU at commit. normals/uvs/mat_ids may be NULL
+ * (geometric normal, zero uv, material 0). mat_ids are local to the mesh and
+ * offset by material_base. */
+int ohpt_scene_begin(ohpt_context* ctx);
+int ohpt_scene_add_mesh(ohpt_context* ctx,
+                        const float* positions, const float* normals, const float* uvs,
+                        uint32_t num_vertices,
+                        const uint32_t* indices, uint32_t num_triangles,
+                        const uint32_t* mat_ids, uint32_t material_base,
+                        const float* world /* 16 floats or NULL */);
+int ohpt_scene_commit(ohpt_context* ctx);
+int ohpt_scene_triangle_count(ohpt_context* ctx, uint64_t* out);
+/* Maximum leaf depth of the committed LBVH (root = 1). */
+int ohpt_scene_tree_depth(ohpt_context* ctx, uint32_t* out);
+
+/* Device-side scene arrays after commit, for inspection/tests. Buffers are
+ * host pointers; sizes in elements as documented in the implementation. */
+int ohpt_scene_export_bvh(ohpt_context* ctx, void* nodes_out /* (n-1)*64 B */,
+                          void* tris_out /* n*48 B */, uint32_t* leaf_prims_out /* n */);
+
+/* Materials and textures: updateRTMaterialParams (rt_build.cpp:313-389)
+ * rewrites materials without touching the BVH. */
+int ohpt_upload_materials(ohpt_context* ctx, const ohpt_material* materials, uint32_t count);
+int ohpt_upload_texture(ohpt_context* ctx, int32_t* out_id, const float* texels,
+                        uint32_t width, uint32_t height, uint32_t channels);
+int ohpt_update_texture(ohpt_context* ctx, int32_t id, const float* texels, int texels_on_device);
+int ohpt_texture_info(ohpt_context* ctx, int32_t id, uint32_t* width, uint32_t* height,
+                      uint32_t* channels);
+
+/* Lights: uploadLightBuffer / updateRTLightParams (light_upload.cpp:145-284). */
+int ohpt_upload_lights(ohpt_context* ctx, const ohpt_light* lights, uint32_t count);
+
+/* Environment: RGBA32F equirect, row 0 = +Y (light_upload.cpp:307-501);
+ * the marginal/conditional CDF is built on the GPU (env_cdf.cpp:13-61).
+ * pixels may be NULL with width=height=0 to clear the environment. */
+int ohpt_upload_env(ohpt_context* ctx, const float* rgba, uint32_t width, uint32_t height);
+int ohpt_set_env_intensity(ohpt_context* ctx, float scale);   /* setEnvIntensityScale */
+int ohpt_export_env_cdf(ohpt_context* ctx, float* marginal /* H */, float* conditional /* W*H */,
+                        float* integral);
+
+/* ---- per-frame state --------------------------------------------------- */
+int ohpt_set_camera(ohpt_context* ctx, const ohpt_camera* cam);
+int ohpt_set_settings(ohpt_context* ctx, const ohpt_settings* settings);
+int ohpt_get_settings(ohpt_context* ctx, ohpt_settings* settings);
+/* resetAccumulation (path_tracer.cpp:189-191): sample index <- seed, count <- 0 */
+int ohpt_reset_accumulation(ohpt_context* ctx);
+/* setRenderSeed (renderer.hpp:160): base sample index after reset */
+int ohpt_set_render_seed(ohpt_context* ctx, uint32_t seed);
+
+/* ---- rendering -------------------------------------------------------------
+ * Traces `spp` samples per pixel into the running-mean accumulation buffer
+ * (pt_raygen_offline.rgen:1211-1262). Synchronous with respect to the host
+ * unless OHPT_RENDER_ASYNC is given; unlike the reference's 3-deep frame ring
+ * (render_dispatch.cpp:139-141) results are visible right after the call. */
+enum {
+    OHPT_RENDER_ASYNC = 1u << 0,    /* return without waiting for the GPU          */
+    OHPT_RENDER_PROFILE = 1u << 1   /* time extend/shade/connect with HIP events    */
+};
+int ohpt_render(ohpt_context* ctx, uint32_t spp, uint32_t render_flags);
+int ohpt_synchronize(ohpt_context* ctx);
+int ohpt_get_stats(ohpt_context* ctx, ohpt_stats* out);
+/* Accumulated HDR beauty, RGBA32F (w = sample count). `on_device` != 0 means
+ * `out` is a device pointer on this context's device. */
+int ohpt_read_accum(ohpt_context* ctx, float* out, int on_device);
+/* Raw per-pixel sums since the last reset: sum = (sum r, g, b, sample count),
+ * sumsq = (sum r^2, g^2, b^2, 0), RGBA32F each. Either may be NULL. Used for
+ * per-pixel variance (no reference counterpart: the reference keeps only the
+ * running mean). */
+int ohpt_read_moments(ohpt_context* ctx, float* sum, float* sumsq, int on_device);
+/* RGBA8: (ACES(x*0.5))^(1/2.2) as pt_raygen_offline.rgen:1338-1354. */
+int ohpt_read_pixels(ohpt_context* ctx, uint8_t* out, int on_device);
+/* First-hit AOVs (readbackHDRBuffers, render_dispatch.cpp:536-626):
+ * albedo RGBA32F, normal RGBA32F (world normal, w=1 on hit), depth F32. Any may be NULL. */
+int ohpt_read_aovs(ohpt_context* ctx, float* albedo, float* normal, float* depth, int on_device);
+
+/* Trace a batch of caller-supplied rays (closest hit). rays: n x 8 floats
+ * (ox,oy,oz,tmin, dx,dy,dz,tmax); hits: n x 4 (t, prim as uint bits, u, v),
+ * prim = 0xFFFFFFFF on miss. prim is the caller's global triangle numbering
+ * (mesh order, then triangle order). Device pointers when on_device != 0. */
+int ohpt_trace_rays(ohpt_context* ctx, const float* rays, float* hits, uint32_t n, int on_device);
+/* Occlusion query: out[i] = 1 if any hit in (tmin, tmax). */
+int ohpt_occluded_rays(ohpt_context* ctx, const float* rays, uint32_t* out, uint32_t n, int on_device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OHPT_H */

@@ -1,0 +1,74 @@
+"""Builds libohpt.so (gfx950) in-tree from ohao_engine_amd/csrc with hipcc.
+
+    python -m ohao_engine_amd.build            # incremental
+    python -m ohao_engine_amd.build --force    # rebuild everything
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+INCLUDE = PKG.parent / "include"
+BUILD = PKG / "_build"
+LIB = PKG / "libohpt.so"
+ARCH = os.environ.get("OHPT_ARCH", "gfx950")
+
+SOURCES = ["api.hip", "lbvh.hip", "traverse.hip", "integrator.hip", "env.hip"]
+HEADERS = ["ohpt_device.h", "ohpt_internal.h"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CFLAGS = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-I", str(INCLUDE),
+          # packed-FP32 instructions are kept out of every kernel
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
+
+
+def _deps_mtime() -> float:
+    files = [CSRC / h for h in HEADERS] + [INCLUDE / "ohpt.h", Path(__file__)]
+    return max(f.stat().st_mtime for f in files if f.exists())
+
+
+def _compile(src: str, force: bool) -> Path:
+    obj = BUILD / (Path(src).stem + ".o")
+    s = CSRC / src
+    if not force and obj.exists() and obj.stat().st_mtime >= max(s.stat().st_mtime, _deps_mtime()):
+        return obj
+    cmd = [HIPCC, *CFLAGS, "-c", str(s), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
+    return obj
+
+
+def build(force: bool = False, verbose: bool = True) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
+    newest = max(o.stat().st_mtime for o in objs)
+    if force or not LIB.exists() or LIB.stat().st_mtime < newest:
+        tmp = LIB.with_suffix(".so.tmp")
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
+        os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    a = ap.parse_args()
+    try:
+        build(a.force)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

@@ -1,0 +1,942 @@
+// C-ABI of the path-tracing core (include/ohpt.h). Host orchestration only:
+// device memory, scene upload + LBVH commit, the wavefront frame loop on one
+// HIP stream, readbacks and timing.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ohpt_internal.h"
+
+using namespace ohpt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+int fail_hip(const char* what, hipError_t e) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return OHPT_ERR_HIP;
+}
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) return fail_hip(#x, e_); \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+struct TexStore {
+    DevBuf data;
+    uint32_t w = 0, h = 0, c = 0;
+};
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+struct ohpt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t W = 0, H = 0;
+
+    // scene staging (host, caller's global triangle order)
+    bool in_scene = false;
+    std::vector<float> h_pos, h_nrm, h_uv;
+    std::vector<uint32_t> h_mat;
+    bool any_nrm = false, any_uv = false;
+    uint32_t n_tris = 0;
+    bool committed = false;
+    uint32_t tree_depth = 0;
+
+    DevBuf d_nodes, d_tris, d_attrs, d_prim, d_slot_of;
+
+    std::vector<ohpt_material> h_materials;
+    DevBuf d_mats;
+    std::vector<TexStore> textures;
+    DevBuf d_texdesc;
+    std::vector<ohpt_light> h_lights;
+    DevBuf d_lights;
+    DevBuf d_emit, d_emit_cdf;
+    uint32_t n_emit = 0;
+    float emit_total = 0.0f;
+
+    DevBuf d_env, d_marg, d_cond, d_rowsum, d_integral;
+    uint32_t env_w = 0, env_h = 0;
+    float env_intensity = 1.0f;
+
+    ohpt_camera cam{};
+    ohpt_settings st{};
+    uint32_t seed = 0;
+    uint32_t sample_index = 0;
+
+    DevBuf d_accum, d_moments, d_aov_albedo, d_aov_normal;
+
+    // path state
+    uint32_t capacity = 0;
+    DevBuf ps_o, ps_d, ps_T, ps_L, ps_ln0, ps_ln1, ps_aov0, ps_aov1, ps_q0, ps_q1, ps_hits, sh_o, sh_d, sh_c,
+        counters, ovf, tmp, tstats;
+    uint32_t ovf_rays = 0;
+
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    std::vector<hipEvent_t> ev_pool;
+    ohpt_stats stats{};
+};
+
+namespace {
+
+DevScene scene_view(ohpt_context* c) {
+    DevScene S{};
+    S.nodes = c->d_nodes.as<float4>();
+    S.tris = c->d_tris.as<float4>();
+    S.attrs = c->d_attrs.as<float4>();
+    S.n_tris = c->committed ? c->n_tris : 0u;
+    S.n_materials = (uint32_t)std::max<size_t>(c->h_materials.size(), 1);
+    S.mats = c->d_mats.as<float4>();
+    S.tex = c->d_texdesc.as<TexDesc>();
+    S.n_tex = (uint32_t)c->textures.size();
+    S.n_lights = (uint32_t)c->h_lights.size();
+    S.lights = c->d_lights.as<float4>();
+    S.emit = c->d_emit.as<float4>();
+    S.emit_cdf = c->d_emit_cdf.as<float>();
+    S.n_emit = c->n_emit;
+    S.emit_total = c->emit_total;
+    S.env = c->d_env.as<float4>();
+    S.env_marg = c->d_marg.as<float>();
+    S.env_cond = c->d_cond.as<float>();
+    S.env_w = c->env_w;
+    S.env_h = c->env_h;
+    S.env_intensity = c->env_intensity;
+    return S;
+}
+
+int alloc_frame(ohpt_context* c) {
+    const size_t npix = (size_t)c->W * c->H;
+    HIPCHK(c->d_accum.ensure(npix * 16));
+    HIPCHK(c->d_moments.ensure(npix * 16));
+    HIPCHK(c->d_aov_albedo.ensure(npix * 16));
+    HIPCHK(c->d_aov_normal.ensure(npix * 16));
+    return OHPT_OK;
+}
+
+int clear_frame(ohpt_context* c) {
+    const size_t npix = (size_t)c->W * c->H;
+    launch_fill_u32(c->d_accum.as<uint32_t>(), 0u, (uint32_t)(npix * 4), c->stream);
+    launch_fill_u32(c->d_moments.as<uint32_t>(), 0u, (uint32_t)(npix * 4), c->stream);
+    launch_fill_u32(c->d_aov_albedo.as<uint32_t>(), 0u, (uint32_t)(npix * 4), c->stream);
+    launch_fill_u32(c->d_aov_normal.as<uint32_t>(), 0u, (uint32_t)(npix * 4), c->stream);
+    HIPCHK(hipGetLastError());
+    return OHPT_OK;
+}
+
+int ensure_paths(ohpt_context* c, uint32_t n) {
+    if (n <= c->capacity) return OHPT_OK;
+    const size_t N = n;
+    HIPCHK(c->ps_o.ensure(N * 16));
+    HIPCHK(c->ps_d.ensure(N * 16));
+    HIPCHK(c->ps_T.ensure(N * 16));
+    HIPCHK(c->ps_L.ensure(N * 16));
+    HIPCHK(c->ps_ln0.ensure(N * 16));
+    HIPCHK(c->ps_ln1.ensure(N * 16));
+    HIPCHK(c->ps_aov0.ensure(N * 16));
+    HIPCHK(c->ps_aov1.ensure(N * 16));
+    HIPCHK(c->ps_q0.ensure(N * 4));
+    HIPCHK(c->ps_q1.ensure(N * 4));
+    HIPCHK(c->ps_hits.ensure(N * 16));
+    HIPCHK(c->sh_o.ensure(2 * N * 16));
+    HIPCHK(c->sh_d.ensure(2 * N * 16));
+    HIPCHK(c->sh_c.ensure(2 * N * 16));
+    HIPCHK(c->counters.ensure(128 * 4));
+    HIPCHK(c->ovf.ensure(2 * N * kStackOvf * 4));
+    c->ovf_rays = 2 * n;
+    c->capacity = n;
+    return OHPT_OK;
+}
+
+PathState path_state(ohpt_context* c) {
+    PathState p{};
+    p.o = c->ps_o.as<float4>();
+    p.d = c->ps_d.as<float4>();
+    p.T = c->ps_T.as<float4>();
+    p.L = c->ps_L.as<float4>();
+    p.ln0 = c->ps_ln0.as<float4>();
+    p.ln1 = c->ps_ln1.as<float4>();
+    p.aov0 = c->ps_aov0.as<float4>();
+    p.aov1 = c->ps_aov1.as<float4>();
+    p.q[0] = c->ps_q0.as<uint32_t>();
+    p.q[1] = c->ps_q1.as<uint32_t>();
+    p.hits = c->ps_hits.as<float4>();
+    p.sh_o = c->sh_o.as<float4>();
+    p.sh_d = c->sh_d.as<float4>();
+    p.sh_c = c->sh_c.as<float4>();
+    p.counters = c->counters.as<uint32_t>();
+    p.capacity = c->capacity;
+    return p;
+}
+
+int upload_materials_dev(ohpt_context* c) {
+    const size_t n = std::max<size_t>(c->h_materials.size(), 1);
+    HIPCHK(c->d_mats.ensure(n * 64));
+    std::vector<float4> packed(n * 4, make_float4(0, 0, 0, 0));
+    for (size_t i = 0; i < c->h_materials.size(); ++i) {
+        const ohpt_material& m = c->h_materials[i];
+        packed[i * 4 + 0] = make_float4(m.base_color[0], m.base_color[1], m.base_color[2], m.roughness);
+        packed[i * 4 + 1] = make_float4(m.metallic, m.specular_f0, m.emissive[0], m.emissive[1]);
+        float a, r, mt;
+        std::memcpy(&a, &m.albedo_tex, 4);
+        std::memcpy(&r, &m.roughness_tex, 4);
+        std::memcpy(&mt, &m.metallic_tex, 4);
+        packed[i * 4 + 2] = make_float4(m.emissive[2], a, r, mt);
+    }
+    if (c->h_materials.empty()) {
+        float none;
+        const int32_t neg = -1;
+        std::memcpy(&none, &neg, 4);
+        packed[0] = make_float4(0.8f, 0.8f, 0.8f, 0.5f);
+        packed[1] = make_float4(0.0f, 0.04f, 0.0f, 0.0f);
+        packed[2] = make_float4(0.0f, none, none, none);
+    }
+    HIPCHK(hipMemcpyAsync(c->d_mats.p, packed.data(), packed.size() * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+// Emitter table: triangles whose material emits, weighted by area * lum(Le).
+int build_emitters(ohpt_context* c) {
+    c->n_emit = 0;
+    c->emit_total = 0.0f;
+    if (!c->committed || c->h_materials.empty()) return OHPT_OK;
+    std::vector<float4> rec;
+    std::vector<double> w;
+    double total = 0.0;
+    for (uint32_t i = 0; i < c->n_tris; ++i) {
+        const uint32_t mi = c->h_mat.empty() ? 0u : c->h_mat[i];
+        if (mi >= c->h_materials.size()) continue;
+        const ohpt_material& m = c->h_materials[mi];
+        const double l = 0.2126 * m.emissive[0] + 0.7152 * m.emissive[1] + 0.0722 * m.emissive[2];
+        if (!(l > 0)) continue;
+        const float* t = &c->h_pos[(size_t)i * 9];
+        const double e1[3] = {(double)t[3] - t[0], (double)t[4] - t[1], (double)t[5] - t[2]};
+        const double e2[3] = {(double)t[6] - t[0], (double)t[7] - t[1], (double)t[8] - t[2]};
+        const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                     cz = e1[0] * e2[1] - e1[1] * e2[0];
+        const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        if (!(area > 0)) continue;
+        float mb;
+        std::memcpy(&mb, &mi, 4);
+        rec.push_back(make_float4(t[0], t[1], t[2], (float)area));
+        rec.push_back(make_float4(t[3], t[4], t[5], mb));
+        rec.push_back(make_float4(t[6], t[7], t[8], 0.0f));
+        w.push_back(area * l);
+        total += area * l;
+    }
+    if (rec.empty()) return OHPT_OK;
+    std::vector<float> cdf(w.size());
+    double acc = 0;
+    for (size_t i = 0; i < w.size(); ++i) { acc += w[i]; cdf[i] = (float)(acc / total); }
+    cdf.back() = 1.0f;
+    HIPCHK(c->d_emit.ensure(rec.size() * 16));
+    HIPCHK(c->d_emit_cdf.ensure(cdf.size() * 4));
+    HIPCHK(hipMemcpyAsync(c->d_emit.p, rec.data(), rec.size() * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_emit_cdf.p, cdf.data(), cdf.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->n_emit = (uint32_t)w.size();
+    c->emit_total = (float)total;
+    return OHPT_OK;
+}
+
+int upload_texdesc(ohpt_context* c) {
+    std::vector<TexDesc> td(std::max<size_t>(c->textures.size(), 1));
+    for (size_t i = 0; i < c->textures.size(); ++i) {
+        td[i].data = c->textures[i].data.as<float>();
+        td[i].w = c->textures[i].w;
+        td[i].h = c->textures[i].h;
+        td[i].c = c->textures[i].c;
+        td[i].pad = 0;
+    }
+    HIPCHK(c->d_texdesc.ensure(td.size() * sizeof(TexDesc)));
+    HIPCHK(hipMemcpyAsync(c->d_texdesc.p, td.data(), td.size() * sizeof(TexDesc), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ohpt_last_error(void) { return g_err.c_str(); }
+int ohpt_abi_version(void) { return OHPT_ABI_VERSION; }
+
+int ohpt_create(ohpt_context** out, int device, uint32_t width, uint32_t height) {
+    if (!out || width == 0 || height == 0) return fail(OHPT_ERR_INVALID_ARG, "ohpt_create: bad arguments");
+    *out = nullptr;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(OHPT_ERR_INVALID_ARG, "ohpt_create: no such device");
+    HIPCHK(hipSetDevice(device));
+    auto* c = new ohpt_context();
+    c->device = device;
+    c->W = width;
+    c->H = height;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
+        delete c;
+        return fail(OHPT_ERR_HIP, "ohpt_create: stream/event creation failed");
+    }
+    // reference offline defaults: maxBounces 4, Sobol sampler, RR after bounce 1 (rt_settings.hpp:50-61)
+    c->st.max_bounces = 4;
+    c->st.rr_start = 2;
+    c->st.sampler = OHPT_SAMPLER_SOBOL_OWEN;
+    c->st.flags = OHPT_FLAG_AOVS;
+    // reference Camera defaults: position (0,0,2.5), yaw -90, pitch 0, fov 45 (camera.cpp:16-22)
+    c->cam = ohpt_camera{{0.0f, 0.0f, 2.5f}, {0.0f, 0.0f, -1.0f}, {1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f},
+                         (float)std::tan(45.0 * M_PI / 360.0)};
+    int r = alloc_frame(c);
+    if (r == OHPT_OK) r = clear_frame(c);
+    if (r == OHPT_OK) r = upload_materials_dev(c);
+    if (r == OHPT_OK) r = upload_texdesc(c);
+    if (r != OHPT_OK) { ohpt_destroy(c); return r; }
+    *out = c;
+    return OHPT_OK;
+}
+
+int ohpt_destroy(ohpt_context* c) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_destroy: null context");
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OHPT_OK;
+}
+
+int ohpt_resize(ohpt_context* c, uint32_t width, uint32_t height) {
+    if (!c || width == 0 || height == 0) return fail(OHPT_ERR_INVALID_ARG, "ohpt_resize: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->W = width;
+    c->H = height;
+    int r = alloc_frame(c);
+    if (r != OHPT_OK) return r;
+    c->sample_index = c->seed;
+    return clear_frame(c);
+}
+
+// ---------------------------------------------------------------- scene
+int ohpt_scene_begin(ohpt_context* c) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_begin: null context");
+    c->in_scene = true;
+    c->h_pos.clear(); c->h_nrm.clear(); c->h_uv.clear(); c->h_mat.clear();
+    c->any_nrm = c->any_uv = false;
+    c->n_tris = 0;
+    return OHPT_OK;
+}
+
+int ohpt_scene_add_mesh(ohpt_context* c, const float* positions, const float* normals, const float* uvs,
+                        uint32_t nv, const uint32_t* indices, uint32_t nt, const uint32_t* mat_ids,
+                        uint32_t material_base, const float* world) {
+    if (!c || !c->in_scene) return fail(OHPT_ERR_STATE, "ohpt_scene_add_mesh: call ohpt_scene_begin first");
+    if (!positions || (nt && !indices)) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_add_mesh: null arrays");
+    if ((uint64_t)c->n_tris + nt >= 0x7FFFFFFFull) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_add_mesh: too many triangles");
+    for (uint64_t i = 0; i < (uint64_t)nt * 3; ++i)
+        if (indices[i] >= nv) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_add_mesh: index out of range");
+    double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    if (world) for (int k = 0; k < 16; ++k) M[k] = world[k];
+    // normal matrix = inverse-transpose of the upper 3x3 (pt_closesthit.rchit:66)
+    const double a = M[0], b = M[4], cc = M[8], d = M[1], e = M[5], f = M[9], g = M[2], h = M[6], i = M[10];
+    const double det = a * (e * i - f * h) - b * (d * i - f * g) + cc * (d * h - e * g);
+    double N[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major N[r*3+col] applied as n' = N n
+    if (std::fabs(det) > 0) {
+        // inverse transpose rows
+        N[0] = (e * i - f * h) / det; N[1] = -(d * i - f * g) / det; N[2] = (d * h - e * g) / det;
+        N[3] = -(b * i - cc * h) / det; N[4] = (a * i - cc * g) / det; N[5] = -(a * h - b * g) / det;
+        N[6] = (b * f - cc * e) / det; N[7] = -(a * f - cc * d) / det; N[8] = (a * e - b * d) / det;
+    }
+    const bool had_nrm = c->any_nrm, had_uv = c->any_uv;
+    const size_t base = c->n_tris;
+    if (normals && !had_nrm && base > 0) {
+        // earlier meshes had no normals: give them their geometric normals
+        c->h_nrm.assign(base * 9, 0.0f);
+        for (size_t t = 0; t < base; ++t) {
+            const float* p = &c->h_pos[t * 9];
+            const float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]}, e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+            const float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            for (int k = 0; k < 3; ++k) for (int q = 0; q < 3; ++q) c->h_nrm[t * 9 + k * 3 + q] = n[q];
+        }
+    }
+    if (normals) c->any_nrm = true;
+    if (uvs && !had_uv) { c->h_uv.assign(base * 6, 0.0f); c->any_uv = true; }
+    c->h_pos.resize((base + nt) * 9);
+    if (c->any_nrm) c->h_nrm.resize((base + nt) * 9);
+    if (c->any_uv) c->h_uv.resize((base + nt) * 6, 0.0f);
+    c->h_mat.resize(base + nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        float* P = &c->h_pos[(base + t) * 9];
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t vi = indices[(size_t)t * 3 + k];
+            const double x = positions[(size_t)vi * 3], y = positions[(size_t)vi * 3 + 1], z = positions[(size_t)vi * 3 + 2];
+            if (world) {
+                P[k * 3 + 0] = (float)(M[0] * x + M[4] * y + M[8] * z + M[12]);
+                P[k * 3 + 1] = (float)(M[1] * x + M[5] * y + M[9] * z + M[13]);
+                P[k * 3 + 2] = (float)(M[2] * x + M[6] * y + M[10] * z + M[14]);
+            } else {
+                P[k * 3 + 0] = (float)x; P[k * 3 + 1] = (float)y; P[k * 3 + 2] = (float)z;
+            }
+            if (c->any_nrm) {
+                float* Nn = &c->h_nrm[(base + t) * 9 + k * 3];
+                if (normals) {
+                    const double nx = normals[(size_t)vi * 3], ny = normals[(size_t)vi * 3 + 1], nz = normals[(size_t)vi * 3 + 2];
+                    double o0 = nx, o1 = ny, o2 = nz;
+                    if (world) {
+                        o0 = N[0] * nx + N[1] * ny + N[2] * nz;
+                        o1 = N[3] * nx + N[4] * ny + N[5] * nz;
+                        o2 = N[6] * nx + N[7] * ny + N[8] * nz;
+                    }
+                    Nn[0] = (float)o0; Nn[1] = (float)o1; Nn[2] = (float)o2;
+                } else {
+                    Nn[0] = Nn[1] = Nn[2] = 0.0f;  // filled with the geometric normal below
+                }
+            }
+            if (c->any_uv) {
+                float* U = &c->h_uv[(base + t) * 6 + k * 2];
+                U[0] = uvs ? uvs[(size_t)vi * 2] : 0.0f;
+                U[1] = uvs ? uvs[(size_t)vi * 2 + 1] : 0.0f;
+            }
+        }
+        if (c->any_nrm && !normals) {
+            const float e1[3] = {P[3] - P[0], P[4] - P[1], P[5] - P[2]}, e2[3] = {P[6] - P[0], P[7] - P[1], P[8] - P[2]};
+            const float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            for (int k = 0; k < 3; ++k) for (int q = 0; q < 3; ++q) c->h_nrm[(base + t) * 9 + k * 3 + q] = n[q];
+        }
+        c->h_mat[base + t] = material_base + (mat_ids ? mat_ids[t] : 0u);
+    }
+    c->n_tris += nt;
+    return OHPT_OK;
+}
+
+int ohpt_scene_commit(ohpt_context* c) {
+    if (!c || !c->in_scene) return fail(OHPT_ERR_STATE, "ohpt_scene_commit: no scene begun");
+    HIPCHK(hipSetDevice(c->device));
+    const uint32_t n = c->n_tris;
+    c->committed = false;
+    if (n == 0) {
+        c->committed = true;
+        c->in_scene = false;
+        c->tree_depth = 0;
+        return build_emitters(c);
+    }
+    DevBuf pos, nrm, uv, mat, scratch;
+    HIPCHK(pos.ensure((size_t)n * 36));
+    HIPCHK(mat.ensure((size_t)n * 4));
+    HIPCHK(hipMemcpyAsync(pos.p, c->h_pos.data(), (size_t)n * 36, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(mat.p, c->h_mat.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    if (c->any_nrm) {
+        HIPCHK(nrm.ensure((size_t)n * 36));
+        HIPCHK(hipMemcpyAsync(nrm.p, c->h_nrm.data(), (size_t)n * 36, hipMemcpyHostToDevice, c->stream));
+    }
+    if (c->any_uv) {
+        HIPCHK(uv.ensure((size_t)n * 24));
+        HIPCHK(hipMemcpyAsync(uv.p, c->h_uv.data(), (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
+    }
+    const uint32_t ni = n > 1 ? n - 1 : 1;
+    HIPCHK(c->d_nodes.ensure((size_t)ni * 64));
+    HIPCHK(c->d_tris.ensure((size_t)n * 48));
+    HIPCHK(c->d_attrs.ensure((size_t)n * 64));
+    HIPCHK(c->d_prim.ensure((size_t)n * 4));
+    HIPCHK(c->d_slot_of.ensure((size_t)n * 4));
+    HIPCHK(scratch.ensure(lbvh_scratch_bytes(n)));
+    LbvhBuffers b{};
+    b.tri_pos = pos.as<float>();
+    b.tri_nrm = c->any_nrm ? nrm.as<float>() : nullptr;
+    b.tri_uv = c->any_uv ? uv.as<float>() : nullptr;
+    b.tri_mat = mat.as<uint32_t>();
+    b.n = n;
+    b.nodes = c->d_nodes.as<float4>();
+    b.tris = c->d_tris.as<float4>();
+    b.attrs = c->d_attrs.as<float4>();
+    b.prim = c->d_prim.as<uint32_t>();
+    b.slot_of = c->d_slot_of.as<uint32_t>();
+    b.scratch = scratch.p;
+    HIPCHK(hipEventRecord(c->ev_start, c->stream));
+    uint32_t depth = 0;
+    hipError_t e = lbvh_build(b, c->stream, &depth);
+    if (e != hipSuccess) return fail_hip("lbvh_build", e);
+    HIPCHK(hipEventRecord(c->ev_stop, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev_stop));
+    c->stats.ms_build = elapsed(c->ev_start, c->ev_stop);
+    c->tree_depth = depth;
+    if (depth > kMaxTreeDepth)
+        return fail(OHPT_ERR_UNSUPPORTED, "ohpt_scene_commit: LBVH depth " + std::to_string(depth) +
+                                              " exceeds the traversal stack");
+    c->committed = true;
+    c->in_scene = false;
+    return build_emitters(c);
+}
+
+int ohpt_scene_triangle_count(ohpt_context* c, uint64_t* out) {
+    if (!c || !out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_triangle_count: null argument");
+    *out = c->committed ? c->n_tris : 0;
+    return OHPT_OK;
+}
+
+int ohpt_scene_export_bvh(ohpt_context* c, void* nodes_out, void* tris_out, uint32_t* leaf_prims_out) {
+    if (!c || !c->committed) return fail(OHPT_ERR_NO_SCENE, "ohpt_scene_export_bvh: no committed scene");
+    HIPCHK(hipSetDevice(c->device));
+    const uint32_t n = c->n_tris;
+    if (n == 0) return OHPT_OK;
+    const uint32_t ni = n > 1 ? n - 1 : 1;
+    if (nodes_out) HIPCHK(hipMemcpyAsync(nodes_out, c->d_nodes.p, (size_t)ni * 64, hipMemcpyDeviceToHost, c->stream));
+    if (tris_out) HIPCHK(hipMemcpyAsync(tris_out, c->d_tris.p, (size_t)n * 48, hipMemcpyDeviceToHost, c->stream));
+    if (leaf_prims_out) HIPCHK(hipMemcpyAsync(leaf_prims_out, c->d_prim.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+// ---------------------------------------------------------------- materials, textures, lights
+int ohpt_upload_materials(ohpt_context* c, const ohpt_material* m, uint32_t count) {
+    if (!c || (count && !m)) return fail(OHPT_ERR_INVALID_ARG, "ohpt_upload_materials: bad arguments");
+    for (uint32_t i = 0; i < count; ++i) {
+        for (int32_t t : {m[i].albedo_tex, m[i].roughness_tex, m[i].metallic_tex})
+            if (t >= (int32_t)c->textures.size())
+                return fail(OHPT_ERR_INVALID_ARG, "ohpt_upload_materials: texture id out of range");
+    }
+    HIPCHK(hipSetDevice(c->device));
+    c->h_materials.assign(m, m + count);
+    int r = upload_materials_dev(c);
+    if (r != OHPT_OK) return r;
+    return build_emitters(c);
+}
+
+int ohpt_upload_texture(ohpt_context* c, int32_t* out_id, const float* texels, uint32_t w, uint32_t h,
+                        uint32_t channels) {
+    if (!c || !out_id || !texels || w == 0 || h == 0 || channels == 0 || channels > 4)
+        return fail(OHPT_ERR_INVALID_ARG, "ohpt_upload_texture: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    TexStore t;
+    t.w = w; t.h = h; t.c = channels;
+    const size_t n = (size_t)w * h;
+    const uint32_t stride = channels >= 2 ? 4u : 1u;
+    std::vector<float> packed(n * stride, 0.0f);
+    for (size_t i = 0; i < n; ++i)
+        for (uint32_t k = 0; k < channels; ++k) packed[i * stride + k] = texels[i * channels + k];
+    if (channels == 3) for (size_t i = 0; i < n; ++i) packed[i * 4 + 3] = 1.0f;
+    HIPCHK(t.data.ensure(packed.size() * 4));
+    HIPCHK(hipMemcpyAsync(t.data.p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->textures.push_back(std::move(t));
+    *out_id = (int32_t)c->textures.size() - 1;
+    return upload_texdesc(c);
+}
+
+// texels: same layout as the upload (w*h*channels floats); on_device = device pointer of that layout
+int ohpt_update_texture(ohpt_context* c, int32_t id, const float* texels, int on_device) {
+    if (!c || !texels || id < 0 || id >= (int32_t)c->textures.size())
+        return fail(OHPT_ERR_INVALID_ARG, "ohpt_update_texture: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    TexStore& t = c->textures[(size_t)id];
+    const size_t n = (size_t)t.w * t.h;
+    const uint32_t stride = t.c >= 2 ? 4u : 1u;
+    std::vector<float> src;
+    if (on_device) {
+        src.resize(n * t.c);
+        HIPCHK(hipMemcpy(src.data(), texels, src.size() * 4, hipMemcpyDeviceToHost));
+        texels = src.data();
+    }
+    std::vector<float> packed(n * stride, 0.0f);
+    for (size_t i = 0; i < n; ++i)
+        for (uint32_t k = 0; k < t.c; ++k) packed[i * stride + k] = texels[i * t.c + k];
+    if (t.c == 3) for (size_t i = 0; i < n; ++i) packed[i * 4 + 3] = 1.0f;
+    HIPCHK(hipMemcpyAsync(t.data.p, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+int ohpt_texture_info(ohpt_context* c, int32_t id, uint32_t* w, uint32_t* h, uint32_t* ch) {
+    if (!c || id < 0 || id >= (int32_t)c->textures.size())
+        return fail(OHPT_ERR_INVALID_ARG, "ohpt_texture_info: bad texture id");
+    const TexStore& t = c->textures[(size_t)id];
+    if (w) *w = t.w;
+    if (h) *h = t.h;
+    if (ch) *ch = t.c;
+    return OHPT_OK;
+}
+
+int ohpt_upload_lights(ohpt_context* c, const ohpt_light* lights, uint32_t count) {
+    if (!c || (count && !lights)) return fail(OHPT_ERR_INVALID_ARG, "ohpt_upload_lights: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    c->h_lights.assign(lights, lights + count);
+    if (count) {
+        HIPCHK(c->d_lights.ensure((size_t)count * sizeof(ohpt_light)));
+        HIPCHK(hipMemcpyAsync(c->d_lights.p, lights, (size_t)count * sizeof(ohpt_light), hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return OHPT_OK;
+}
+
+int ohpt_upload_env(ohpt_context* c, const float* rgba, uint32_t w, uint32_t h) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_upload_env: null context");
+    HIPCHK(hipSetDevice(c->device));
+    if (!rgba || w == 0 || h == 0) {
+        c->env_w = c->env_h = 0;
+        return OHPT_OK;
+    }
+    const size_t n = (size_t)w * h;
+    HIPCHK(c->d_env.ensure(n * 16));
+    HIPCHK(c->d_cond.ensure(n * 4));
+    HIPCHK(c->d_marg.ensure((size_t)h * 4));
+    HIPCHK(c->d_rowsum.ensure((size_t)h * 4));
+    HIPCHK(c->d_integral.ensure(4));
+    HIPCHK(hipMemcpyAsync(c->d_env.p, rgba, n * 16, hipMemcpyHostToDevice, c->stream));
+    hipError_t e = env_cdf_build(c->d_env.as<float4>(), w, h, c->d_marg.as<float>(), c->d_cond.as<float>(),
+                                 c->d_rowsum.as<float>(), c->d_integral.as<float>(), c->stream);
+    if (e != hipSuccess) return fail_hip("env_cdf_build", e);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->env_w = w;
+    c->env_h = h;
+    return OHPT_OK;
+}
+
+int ohpt_set_env_intensity(ohpt_context* c, float scale) {
+    if (!c || !(scale >= 0.0f)) return fail(OHPT_ERR_INVALID_ARG, "ohpt_set_env_intensity: bad arguments");
+    c->env_intensity = scale;
+    return OHPT_OK;
+}
+
+int ohpt_export_env_cdf(ohpt_context* c, float* marginal, float* conditional, float* integral) {
+    if (!c || c->env_w == 0) return fail(OHPT_ERR_STATE, "ohpt_export_env_cdf: no environment");
+    HIPCHK(hipSetDevice(c->device));
+    if (marginal) HIPCHK(hipMemcpyAsync(marginal, c->d_marg.p, (size_t)c->env_h * 4, hipMemcpyDeviceToHost, c->stream));
+    if (conditional)
+        HIPCHK(hipMemcpyAsync(conditional, c->d_cond.p, (size_t)c->env_w * c->env_h * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    if (integral) HIPCHK(hipMemcpyAsync(integral, c->d_integral.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+// ---------------------------------------------------------------- per-frame state
+int ohpt_set_camera(ohpt_context* c, const ohpt_camera* cam) {
+    if (!c || !cam || !(cam->tan_half_fov_y > 0.0f)) return fail(OHPT_ERR_INVALID_ARG, "ohpt_set_camera: bad camera");
+    c->cam = *cam;
+    return OHPT_OK;
+}
+
+int ohpt_set_settings(ohpt_context* c, const ohpt_settings* s) {
+    if (!c || !s) return fail(OHPT_ERR_INVALID_ARG, "ohpt_set_settings: null argument");
+    if (s->max_bounces > 60) return fail(OHPT_ERR_INVALID_ARG, "ohpt_set_settings: max_bounces > 60");
+    if (s->sampler > OHPT_SAMPLER_SOBOL_OWEN) return fail(OHPT_ERR_INVALID_ARG, "ohpt_set_settings: unknown sampler");
+    c->st = *s;
+    return OHPT_OK;
+}
+
+int ohpt_get_settings(ohpt_context* c, ohpt_settings* s) {
+    if (!c || !s) return fail(OHPT_ERR_INVALID_ARG, "ohpt_get_settings: null argument");
+    *s = c->st;
+    return OHPT_OK;
+}
+
+int ohpt_reset_accumulation(ohpt_context* c) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_reset_accumulation: null context");
+    HIPCHK(hipSetDevice(c->device));
+    c->sample_index = c->seed;
+    return clear_frame(c);
+}
+
+int ohpt_set_render_seed(ohpt_context* c, uint32_t seed) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_set_render_seed: null context");
+    c->seed = seed;
+    return OHPT_OK;
+}
+
+// ---------------------------------------------------------------- rendering
+int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_render: null context");
+    if (!c->committed) return fail(OHPT_ERR_NO_SCENE, "ohpt_render: no committed scene");
+    if (spp == 0) return OHPT_OK;
+    HIPCHK(hipSetDevice(c->device));
+    const uint64_t npix = (uint64_t)c->W * c->H;
+    const uint64_t target = std::max<uint64_t>(npix, 1ull << 22);
+    const uint32_t per_batch = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(spp, target / npix));
+    if (npix * per_batch > 0x7FFFFFFFull) return fail(OHPT_ERR_INVALID_ARG, "ohpt_render: image too large");
+    int r = ensure_paths(c, (uint32_t)(npix * per_batch));
+    if (r != OHPT_OK) return r;
+    const DevScene S = scene_view(c);
+    const PathState ps = path_state(c);
+    const bool profile = (render_flags & 2u) != 0;
+    std::vector<hipEvent_t> evs;
+    auto mark = [&](hipEvent_t* out) -> int {
+        if (!profile) return OHPT_OK;
+        const size_t k = evs.size();
+        if (k >= c->ev_pool.size()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c->ev_pool.push_back(e);
+        }
+        *out = c->ev_pool[k];
+        evs.push_back(*out);
+        HIPCHK(hipEventRecord(*out, c->stream));
+        return OHPT_OK;
+    };
+    struct Seg { hipEvent_t a, b; int kind; };
+    std::vector<Seg> segs;
+    unsigned long long* tstats = nullptr;
+    if (profile) {
+        HIPCHK(c->tstats.ensure(16));
+        tstats = c->tstats.as<unsigned long long>();
+        launch_fill_u32(c->tstats.as<uint32_t>(), 0u, 4, c->stream);
+    }
+    uint32_t n_extend = 0;
+    HIPCHK(hipEventRecord(c->ev_start, c->stream));
+    uint64_t rays_ext = 0, rays_sh = 0;
+    std::vector<uint32_t> host_counters(128);
+    uint32_t remaining = spp;
+    while (remaining > 0) {
+        const uint32_t batch = std::min(remaining, per_batch);
+        FrameParams f{};
+        f.cam = c->cam;
+        f.width = c->W;
+        f.height = c->H;
+        f.n_paths = (uint32_t)(npix * batch);
+        f.sample_base = c->sample_index;
+        f.max_bounces = c->st.max_bounces;
+        f.rr_start = c->st.rr_start;
+        f.sampler = c->st.sampler;
+        f.flags = c->st.flags;
+        f.firefly_clamp = c->st.firefly_clamp;
+        f.ray_eps = c->st.ray_eps;
+        launch_fill_u32(ps.counters, 0u, 128, c->stream);
+        launch_generate(f, ps, c->stream);
+        for (uint32_t b = 0; b <= f.max_bounces; ++b) {
+            Seg sg{};
+            if ((r = mark(&sg.a))) return r;
+            launch_extend(S, ps.q[b & 1u], ps.counters + b, f.n_paths, ps.o, ps.d, ps.hits, c->ovf.as<uint32_t>(),
+                          tstats, c->stream);
+            ++n_extend;
+            if ((r = mark(&sg.b))) return r;
+            if (profile) { sg.kind = 0; segs.push_back(sg); }
+            if ((r = mark(&sg.a))) return r;
+            launch_shade(S, f, ps, b, c->stream);
+            if ((r = mark(&sg.b))) return r;
+            if (profile) { sg.kind = 1; segs.push_back(sg); }
+            if (b < f.max_bounces) {
+                if ((r = mark(&sg.a))) return r;
+                launch_connect(S, ps.counters + kShadowCounterBase + b, 2u * f.n_paths, ps.sh_o, ps.sh_d, ps.sh_c,
+                               ps.ln0, ps.ln1, c->ovf.as<uint32_t>(), c->stream);
+                if ((r = mark(&sg.b))) return r;
+                if (profile) { sg.kind = 2; segs.push_back(sg); }
+            }
+        }
+        launch_accumulate(f, ps, c->d_accum.as<float4>(), c->d_moments.as<float4>(), c->d_aov_albedo.as<float4>(),
+                          c->d_aov_normal.as<float4>(), c->stream);
+        HIPCHK(hipGetLastError());
+        if (!(render_flags & OHPT_RENDER_ASYNC)) {
+            HIPCHK(hipMemcpyAsync(host_counters.data(), ps.counters, 128 * 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (uint32_t b = 0; b <= f.max_bounces; ++b) rays_ext += host_counters[b];
+            for (uint32_t b = 0; b < f.max_bounces; ++b) rays_sh += host_counters[kShadowCounterBase + b];
+        }
+        c->sample_index += batch;
+        remaining -= batch;
+    }
+    HIPCHK(hipEventRecord(c->ev_stop, c->stream));
+    if (!(render_flags & OHPT_RENDER_ASYNC)) {
+        HIPCHK(hipEventSynchronize(c->ev_stop));
+        c->stats.ms_total = elapsed(c->ev_start, c->ev_stop);
+        c->stats.ms_extend = c->stats.ms_shade = c->stats.ms_connect = 0.0;
+        for (const Seg& sg : segs) {
+            const double ms = elapsed(sg.a, sg.b);
+            if (sg.kind == 0) c->stats.ms_extend += ms;
+            else if (sg.kind == 1) c->stats.ms_shade += ms;
+            else c->stats.ms_connect += ms;
+        }
+        c->stats.rays_extend = rays_ext;
+        c->stats.rays_shadow = rays_sh;
+        c->stats.paths = npix * spp;
+        c->stats.extend_launches = n_extend;
+        c->stats.node_visits = c->stats.tri_tests = 0;
+        if (profile) {
+            unsigned long long h[2] = {0, 0};
+            HIPCHK(hipMemcpy(h, tstats, 16, hipMemcpyDeviceToHost));
+            c->stats.node_visits = h[0];
+            c->stats.tri_tests = h[1];
+        }
+    }
+    return OHPT_OK;
+}
+
+int ohpt_synchronize(ohpt_context* c) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_synchronize: null context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+int ohpt_get_stats(ohpt_context* c, ohpt_stats* out) {
+    if (!c || !out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_get_stats: null argument");
+    *out = c->stats;
+    return OHPT_OK;
+}
+
+extern "C++" {
+namespace {
+template <class F>
+int produce(ohpt_context* c, void* out, size_t bytes, int on_device, F&& kernel) {
+    HIPCHK(hipSetDevice(c->device));
+    if (on_device) {
+        kernel(out);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return OHPT_OK;
+    }
+    HIPCHK(c->tmp.ensure(bytes));
+    kernel(c->tmp.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, c->tmp.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int ohpt_read_accum(ohpt_context* c, float* out, int on_device) {
+    if (!c || !out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_read_accum: null argument");
+    const uint32_t n = c->W * c->H;
+    return produce(c, out, (size_t)n * 16, on_device, [&](void* dst) {
+        launch_resolve(c->d_accum.as<float4>(), reinterpret_cast<float4*>(dst), n, c->stream);
+    });
+}
+
+int ohpt_read_moments(ohpt_context* c, float* sum, float* sumsq, int on_device) {
+    if (!c || (!sum && !sumsq)) return fail(OHPT_ERR_INVALID_ARG, "ohpt_read_moments: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t bytes = (size_t)c->W * c->H * 16;
+    const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (sum) HIPCHK(hipMemcpyAsync(sum, c->d_accum.p, bytes, k, c->stream));
+    if (sumsq) HIPCHK(hipMemcpyAsync(sumsq, c->d_moments.p, bytes, k, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+int ohpt_read_pixels(ohpt_context* c, uint8_t* out, int on_device) {
+    if (!c || !out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_read_pixels: null argument");
+    const uint32_t n = c->W * c->H;
+    return produce(c, out, (size_t)n * 4, on_device, [&](void* dst) {
+        launch_tonemap(c->d_accum.as<float4>(), reinterpret_cast<uint8_t*>(dst), n, c->stream);
+    });
+}
+
+int ohpt_read_aovs(ohpt_context* c, float* albedo, float* normal, float* depth, int on_device) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_read_aovs: null context");
+    HIPCHK(hipSetDevice(c->device));
+    const uint32_t n = c->W * c->H;
+    if (on_device) {
+        launch_resolve_aovs(c->d_accum.as<float4>(), c->d_aov_albedo.as<float4>(), c->d_aov_normal.as<float4>(),
+                            albedo, normal, depth, n, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return OHPT_OK;
+    }
+    HIPCHK(c->tmp.ensure((size_t)n * 36));
+    float* ta = c->tmp.as<float>();
+    float* tn = ta + (size_t)n * 4;
+    float* td = tn + (size_t)n * 4;
+    launch_resolve_aovs(c->d_accum.as<float4>(), c->d_aov_albedo.as<float4>(), c->d_aov_normal.as<float4>(), ta, tn,
+                        td, n, c->stream);
+    HIPCHK(hipGetLastError());
+    if (albedo) HIPCHK(hipMemcpyAsync(albedo, ta, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
+    if (normal) HIPCHK(hipMemcpyAsync(normal, tn, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
+    if (depth) HIPCHK(hipMemcpyAsync(depth, td, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+// ---------------------------------------------------------------- ray batches
+extern "C++" {
+namespace {
+int ray_batch(ohpt_context* c, const float* rays, void* out, uint32_t n, int on_device, bool occl) {
+    if (!c || !rays || !out) return fail(OHPT_ERR_INVALID_ARG, "ray batch: null argument");
+    if (!c->committed) return fail(OHPT_ERR_NO_SCENE, "ray batch: no committed scene");
+    if (n == 0) return OHPT_OK;
+    HIPCHK(hipSetDevice(c->device));
+    DevBuf dr, dout, dovf;
+    HIPCHK(dovf.ensure((size_t)n * kStackOvf * 4));
+    const float4* r = reinterpret_cast<const float4*>(rays);
+    void* o = out;
+    const size_t out_bytes = (size_t)n * (occl ? 4 : 16);
+    if (!on_device) {
+        HIPCHK(dr.ensure((size_t)n * 32));
+        HIPCHK(dout.ensure(out_bytes));
+        HIPCHK(hipMemcpyAsync(dr.p, rays, (size_t)n * 32, hipMemcpyHostToDevice, c->stream));
+        r = dr.as<float4>();
+        o = dout.p;
+    }
+    const DevScene S = scene_view(c);
+    if (occl) launch_occluded_rays(S, r, reinterpret_cast<uint32_t*>(o), n, dovf.as<uint32_t>(), c->stream);
+    else launch_trace_rays(S, r, reinterpret_cast<float4*>(o), n, dovf.as<uint32_t>(), c->stream);
+    HIPCHK(hipGetLastError());
+    if (!on_device) HIPCHK(hipMemcpyAsync(out, dout.p, out_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int ohpt_trace_rays(ohpt_context* c, const float* rays, float* hits, uint32_t n, int on_device) {
+    return ray_batch(c, rays, hits, n, on_device, false);
+}
+
+int ohpt_occluded_rays(ohpt_context* c, const float* rays, uint32_t* out, uint32_t n, int on_device) {
+    return ray_batch(c, rays, out, n, on_device, true);
+}
+
+int ohpt_scene_tree_depth(ohpt_context* c, uint32_t* out) {
+    if (!c || !out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_tree_depth: null argument");
+    *out = c->tree_depth;
+    return OHPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,398 @@
+// Wavefront path-tracing kernels: generate -> (extend -> shade -> connect)* -> accumulate.
+//
+// The estimator matches the per-pixel integrator of
+// shaders/rt/pt_raygen_offline.rgen:129-1262 in structure (camera ray, NEE to
+// lights and the environment with balance-heuristic MIS, BSDF sampling,
+// Russian roulette, running accumulation) with a single GGX-VNDF/Lambert
+// chain instead of the reference's dual bounce-0 chains.
+//
+// Path state lives in SoA arrays in HBM indexed by path id
+// pid = k*W*H + pixel (k = sample within the batch). Extend queues hold path
+// ids, compacted with wave-aggregated atomics; shadow rays go to a separate
+// queue and their radiance lands in per-path light/env accumulators (one
+// writer per bounce), so the image is independent of queue order.
+#include "ohpt_internal.h"
+
+namespace ohpt {
+namespace {
+
+constexpr uint32_t kShadeBlock = 256;
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ f3 camera_dir(const FrameParams& f, uint32_t px, uint32_t py, float jx, float jy) {
+    const float ndcx = ((float)px + 0.5f + jx) / (float)f.width * 2.0f - 1.0f;
+    const float ndcy = ((float)py + 0.5f + jy) / (float)f.height * 2.0f - 1.0f;
+    const float ty = f.cam.tan_half_fov_y, tx = ty * ((float)f.width / (float)f.height);
+    const f3 fwd = mk(f.cam.forward[0], f.cam.forward[1], f.cam.forward[2]);
+    const f3 right = mk(f.cam.right[0], f.cam.right[1], f.cam.right[2]);
+    const f3 up = mk(f.cam.up[0], f.cam.up[1], f.cam.up[2]);
+    return normalize(fwd + right * (ndcx * tx) - up * (ndcy * ty));
+}
+
+__global__ __launch_bounds__(256) void k_generate(FrameParams f, float4* __restrict__ o, float4* __restrict__ d,
+                                                  float4* __restrict__ T, float4* __restrict__ L,
+                                                  float4* __restrict__ ln0, float4* __restrict__ ln1,
+                                                  uint32_t* __restrict__ q, uint32_t* __restrict__ counters) {
+    const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
+    if (pid == 0) counters[0] = f.n_paths;
+    if (pid >= f.n_paths) return;
+    const uint32_t npix = f.width * f.height;
+    const uint32_t pixel = pid % npix, k = pid / npix;
+    const uint32_t px = pixel % f.width, py = pixel / f.width;
+    float u[2];
+    draw_dims(f.sampler, px, py, f.width, f.sample_base + k, 0, 2, u);
+    const f3 dir = camera_dir(f, px, py, u[0] - 0.5f, u[1] - 0.5f);
+    o[pid] = make_float4(f.cam.position[0], f.cam.position[1], f.cam.position[2], 0.0f);
+    d[pid] = f4(dir, 0.0f);
+    T[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    L[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ln0[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    ln1[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    q[pid] = pid;
+}
+
+struct SurfaceHit {
+    f3 p, ng, ns;
+    float u, v;
+    uint32_t mat;
+    bool front;
+};
+
+__device__ __forceinline__ SurfaceHit surface(const DevScene& S, uint32_t slot, float b1, float b2, f3 d) {
+    const float4* tp = S.tris + (size_t)slot * 3;
+    const f3 v0 = xyz(tp[0]), v1 = xyz(tp[1]), v2 = xyz(tp[2]);
+    const float4* ap = S.attrs + (size_t)slot * 4;
+    const float4 a0 = ap[0], a1 = ap[1], a2 = ap[2], a3 = ap[3];
+    const float w = 1.0f - b1 - b2;
+    SurfaceHit s;
+    s.p = v0 * w + v1 * b1 + v2 * b2;
+    f3 ng = normalize(cross(v1 - v0, v2 - v0));
+    const f3 n0 = mk(a0.x, a0.y, a0.z), n1 = mk(a0.w, a1.x, a1.y), n2 = mk(a1.z, a1.w, a2.x);
+    const f3 ni = n0 * w + n1 * b1 + n2 * b2;
+    f3 ns = dot(ni, ni) > 1e-12f ? normalize(ni) : ng;
+    s.u = a2.y * w + a2.w * b1 + a3.y * b2;
+    s.v = a2.z * w + a3.x * b1 + a3.z * b2;
+    s.mat = __float_as_uint(a3.w);
+    s.front = dot(ng, d) < 0.0f;
+    if (!s.front) { ng = -ng; ns = -ns; }
+    if (!(dot(ns, ng) > 0.0f) || !(dot(ns, -d) > 1e-6f)) ns = ng;
+    s.ng = ng;
+    s.ns = ns;
+    return s;
+}
+
+__device__ __forceinline__ float offset_eps(const FrameParams& f, f3 p) {
+    const float e = f.ray_eps > 0.0f ? f.ray_eps : 1e-4f;
+    return e * fmaxf(1.0f, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+}
+
+// first index with cdf[i] > u
+__device__ __forceinline__ uint32_t cdf_upper(const float* cdf, uint32_t n, float u) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid + 1; else hi = mid;
+    }
+    return lo < n ? lo : n - 1;
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f, PathState ps, uint32_t bounce) {
+    const uint32_t* __restrict__ qin = ps.q[bounce & 1u];
+    uint32_t* __restrict__ qout = ps.q[(bounce + 1u) & 1u];
+    const uint32_t count = ps.counters[bounce];
+    const uint32_t j = blockIdx.x * kShadeBlock + threadIdx.x;
+    const bool active = j < count;
+    bool emit_next = false, emit_l = false, emit_e = false;
+    uint32_t pid = 0;
+    float4 next_o = make_float4(0, 0, 0, 0), next_d = make_float4(0, 0, 0, 0);
+    float4 shl_o, shl_d, shl_c, she_o, she_d, she_c;
+    if (active) {
+        pid = qin[j];
+        const float4 hit = ps.hits[j];
+        const float4 dd = ps.d[pid];
+        const f3 d = xyz(dd);
+        const float prev_pdf = dd.w;
+        float4 T4 = ps.T[pid];
+        f3 T = xyz(T4);
+        float4 L4 = ps.L[pid];
+        f3 L = xyz(L4);
+        const bool light_nee = !(f.flags & OHPT_FLAG_NO_LIGHT_NEE);
+        const bool env_nee = !(f.flags & OHPT_FLAG_NO_ENV_NEE) && S.env_w > 0;
+        const uint32_t NA = S.n_lights;
+        const uint32_t NL = NA + (S.n_emit > 0 ? 1u : 0u);
+        const uint32_t slot = __float_as_uint(hit.y);
+        if (slot == kMiss) {
+            if (S.env_w > 0) {
+                const f3 Le = env_eval(S, d);
+                const float w = (bounce == 0 || !env_nee) ? 1.0f : balance(prev_pdf, env_pdf(S, d));
+                L = L + T * Le * w;
+            }
+            if (bounce == 0 && (f.flags & OHPT_FLAG_AOVS)) {
+                ps.aov0[pid] = make_float4(0, 0, 0, 0);
+                ps.aov1[pid] = make_float4(0, 0, 0, 0);
+            }
+        } else {
+            const SurfaceHit s = surface(S, slot, hit.z, hit.w, d);
+            const f3 Le = material_emission(S, s.mat);
+            if (s.front && maxc(Le) > 0.0f) {
+                float w = 1.0f;
+                if (bounce > 0 && light_nee && NL > 0 && S.n_emit > 0) {
+                    const float cl = -dot(s.ng, d);
+                    const float pa = lum(Le) / ((float)NL * S.emit_total);
+                    w = balance(prev_pdf, pa * hit.x * hit.x / fmaxf(cl, 1e-30f));
+                }
+                L = L + T * Le * w;
+            }
+            const Material m = load_material(S, s.mat, s.u, s.v);
+            if (bounce == 0 && (f.flags & OHPT_FLAG_AOVS)) {
+                ps.aov0[pid] = f4(m.albedo, hit.x);
+                ps.aov1[pid] = f4(s.ns, 1.0f);
+            }
+            if (bounce < f.max_bounces) {
+                const uint32_t npix = f.width * f.height;
+                const uint32_t pixel = pid % npix, ks = pid / npix;
+                float u[kDimsPerVertex];
+                draw_dims(f.sampler, pixel % f.width, pixel / f.width, f.width, f.sample_base + ks, 1u + bounce,
+                          kDimsPerVertex, u);
+                const f3 wo = -d;
+                const float eps = offset_eps(f, s.p);
+                const f3 po = s.p + s.ng * eps;
+                // ---- NEE: one light
+                if (light_nee && NL > 0) {
+                    uint32_t k = (uint32_t)(u[0] * (float)NL);
+                    if (k > NL - 1) k = NL - 1;
+                    if (k < NA) {
+                        f3 wi, Lw;
+                        float dist;
+                        if (sample_analytic(S.lights + (size_t)k * 5, s.p, u[1], u[2], wi, dist, Lw)) {
+                            const float ci = dot(s.ns, wi);
+                            if (ci > 0.0f && dot(s.ng, wi) > 0.0f) {
+                                const f3 fb = bsdf_eval(m, s.ns, wo, wi);
+                                if (maxc(fb) > 0.0f) {
+                                    const float tm = isinf(dist) ? INFINITY : dist * (1.0f - 1e-4f) - eps;
+                                    shl_o = f4(po, tm);
+                                    shl_d = f4(wi, __uint_as_float(pid * 2u));
+                                    shl_c = f4(T * fb * Lw * (ci * (float)NL), 0.0f);
+                                    emit_l = true;
+                                }
+                            }
+                        }
+                    } else {
+                        float ut = fminf(fmaxf(u[0] * (float)NL - (float)k, 0.0f), 0.99999994f);
+                        const uint32_t ei = cdf_upper(S.emit_cdf, S.n_emit, ut);
+                        const float4* ep = S.emit + (size_t)ei * 3;
+                        const float4 e0 = ep[0], e1 = ep[1], e2 = ep[2];
+                        const f3 a0 = xyz(e0), a1 = xyz(e1), a2 = xyz(e2);
+                        const float su = sqrtf(u[1]);
+                        const float b0 = 1.0f - su, b1 = u[2] * su, b2 = 1.0f - b0 - b1;
+                        const f3 x = a0 * b0 + a1 * b1 + a2 * b2;
+                        const f3 nl = normalize(cross(a1 - a0, a2 - a0));
+                        const f3 toL = x - s.p;
+                        const float dist = length(toL);
+                        if (dist > 0.0f && e0.w > 0.0f) {
+                            const f3 wi = toL * (1.0f / dist);
+                            const float cl = -dot(nl, wi), ci = dot(s.ns, wi);
+                            if (cl > 0.0f && ci > 0.0f && dot(s.ng, wi) > 0.0f) {
+                                const f3 Le2 = material_emission(S, __float_as_uint(e1.w));
+                                const float pl = lum(Le2) / ((float)NL * S.emit_total) * dist * dist / cl;
+                                const f3 fb = bsdf_eval(m, s.ns, wo, wi);
+                                if (maxc(fb) > 0.0f && pl > 0.0f) {
+                                    const float w = balance(pl, bsdf_pdf(m, s.ns, wo, wi));
+                                    shl_o = f4(po, dist * (1.0f - 1e-4f) - eps);
+                                    shl_d = f4(wi, __uint_as_float(pid * 2u));
+                                    shl_c = f4(T * fb * Le2 * (ci * w / pl), 0.0f);
+                                    emit_l = true;
+                                }
+                            }
+                        }
+                    }
+                }
+                // ---- NEE: environment
+                if (env_nee) {
+                    f3 wi;
+                    float pe;
+                    if (env_sample(S, u[3], u[4], wi, pe)) {
+                        const float ci = dot(s.ns, wi);
+                        if (ci > 0.0f && dot(s.ng, wi) > 0.0f) {
+                            const f3 fb = bsdf_eval(m, s.ns, wo, wi);
+                            if (maxc(fb) > 0.0f) {
+                                const float w = balance(pe, bsdf_pdf(m, s.ns, wo, wi));
+                                she_o = f4(po, INFINITY);
+                                she_d = f4(wi, __uint_as_float(pid * 2u + 1u));
+                                she_c = f4(T * fb * env_eval(S, wi) * (ci * w / pe), 0.0f);
+                                emit_e = true;
+                            }
+                        }
+                    }
+                }
+                // ---- Russian roulette, then BSDF sampling
+                bool alive = true;
+                if (bounce >= f.rr_start) {
+                    const float qc = fminf(1.0f, maxc(T));
+                    if (!(u[5] < qc)) alive = false;
+                    else T = T * (1.0f / qc);
+                }
+                if (alive) {
+                    f3 wi;
+                    if (bsdf_sample(m, s.ns, wo, u[6], u[7], u[8], wi)) {
+                        const float ci = dot(s.ns, wi);
+                        const float pdf = bsdf_pdf(m, s.ns, wo, wi);
+                        if (ci > 0.0f && dot(s.ng, wi) > 0.0f && pdf > 0.0f) {
+                            const f3 fb = bsdf_eval(m, s.ns, wo, wi);
+                            T = T * fb * (ci / pdf);
+                            if (maxc(T) > 0.0f) {
+                                next_o = f4(po, 0.0f);
+                                next_d = f4(wi, pdf);
+                                emit_next = true;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        ps.L[pid] = f4(L, L4.w);
+        if (emit_next) {
+            ps.T[pid] = f4(T, T4.w);
+            ps.o[pid] = next_o;
+            ps.d[pid] = next_d;
+        }
+    }
+    // compaction (all lanes participate in the ballots)
+    uint32_t* cnt_sh = ps.counters + kShadowCounterBase + bounce;
+    const uint32_t sl = wave_append(cnt_sh, emit_l);
+    const uint32_t se = wave_append(cnt_sh, emit_e);
+    const uint32_t sn = wave_append(ps.counters + bounce + 1, emit_next);
+    if (emit_l) { ps.sh_o[sl] = shl_o; ps.sh_d[sl] = shl_d; ps.sh_c[sl] = shl_c; }
+    if (emit_e) { ps.sh_o[se] = she_o; ps.sh_d[se] = she_d; ps.sh_c[se] = she_c; }
+    if (emit_next) qout[sn] = pid;
+}
+
+__global__ __launch_bounds__(256) void k_accumulate(FrameParams f, const float4* __restrict__ L,
+                                                    const float4* __restrict__ ln0, const float4* __restrict__ ln1,
+                                                    const float4* __restrict__ aov0, const float4* __restrict__ aov1,
+                                                    float4* __restrict__ accum, float4* __restrict__ moments,
+                                                    float4* __restrict__ aov_albedo, float4* __restrict__ aov_normal) {
+    const uint32_t npix = f.width * f.height;
+    const uint32_t pixel = blockIdx.x * 256 + threadIdx.x;
+    if (pixel >= npix) return;
+    const uint32_t spp = f.n_paths / npix;
+    float sx = 0, sy = 0, sz = 0, qx = 0, qy = 0, qz = 0;
+    float ax = 0, ay = 0, az = 0, dep = 0, nx = 0, ny = 0, nz = 0, nh = 0;
+    const bool aovs = (f.flags & OHPT_FLAG_AOVS) != 0;
+    for (uint32_t k = 0; k < spp; ++k) {
+        const uint32_t pid = k * npix + pixel;
+        const float4 a = L[pid], b = ln0[pid], c = ln1[pid];
+        float rx = a.x + b.x + c.x, ry = a.y + b.y + c.y, rz = a.z + b.z + c.z;
+        if ((f.flags & OHPT_FLAG_FIREFLY_CLAMP) && f.firefly_clamp > 0.0f) {
+            const float l = 0.2126f * rx + 0.7152f * ry + 0.0722f * rz;
+            if (l > f.firefly_clamp) { const float s = f.firefly_clamp / l; rx *= s; ry *= s; rz *= s; }
+        }
+        if (!(rx == rx) || !(ry == ry) || !(rz == rz)) { rx = ry = rz = 0.0f; }
+        sx += rx; sy += ry; sz += rz;
+        qx += rx * rx; qy += ry * ry; qz += rz * rz;
+        if (aovs) {
+            const float4 p0 = aov0[pid], p1 = aov1[pid];
+            ax += p0.x; ay += p0.y; az += p0.z; dep += p0.w;
+            nx += p1.x; ny += p1.y; nz += p1.z; nh += p1.w;
+        }
+    }
+    float4 acc = accum[pixel];
+    acc.x += sx; acc.y += sy; acc.z += sz; acc.w += (float)spp;
+    accum[pixel] = acc;
+    float4 mo = moments[pixel];
+    mo.x += qx; mo.y += qy; mo.z += qz;
+    moments[pixel] = mo;
+    if (aovs) {
+        float4 aa = aov_albedo[pixel];
+        aa.x += ax; aa.y += ay; aa.z += az; aa.w += dep;
+        aov_albedo[pixel] = aa;
+        float4 an = aov_normal[pixel];
+        an.x += nx; an.y += ny; an.z += nz; an.w += nh;
+        aov_normal[pixel] = an;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_resolve(const float4* __restrict__ acc, float4* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = acc[i];
+    const float inv = a.w > 0.0f ? 1.0f / a.w : 0.0f;
+    out[i] = make_float4(a.x * inv, a.y * inv, a.z * inv, a.w);
+}
+
+// (ACES(x*0.5))^(1/2.2) -> RGBA8, pt_raygen_offline.rgen:102-105, :1338-1354
+__device__ __forceinline__ float aces(float x) {
+    return fminf(1.0f, fmaxf(0.0f, (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f)));
+}
+__global__ __launch_bounds__(256) void k_tonemap(const float4* __restrict__ acc, uint8_t* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = acc[i];
+    const float inv = a.w > 0.0f ? 1.0f / a.w : 0.0f;
+    const float c[3] = {a.x * inv, a.y * inv, a.z * inv};
+    uchar4 o;
+    uint8_t v[3];
+    for (int k = 0; k < 3; ++k) {
+        const float t = powf(aces(c[k] * 0.5f), 1.0f / 2.2f);
+        v[k] = (uint8_t)fminf(255.0f, fmaxf(0.0f, t * 255.0f + 0.5f));
+    }
+    o.x = v[0]; o.y = v[1]; o.z = v[2]; o.w = 255;
+    reinterpret_cast<uchar4*>(out)[i] = o;
+}
+
+__global__ __launch_bounds__(256) void k_resolve_aovs(const float4* __restrict__ acc, const float4* __restrict__ aa,
+                                                      const float4* __restrict__ an, float* __restrict__ albedo,
+                                                      float* __restrict__ normal, float* __restrict__ depth,
+                                                      uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float cnt = acc[i].w;
+    const float inv = cnt > 0.0f ? 1.0f / cnt : 0.0f;
+    const float4 a = aa[i], b = an[i];
+    if (albedo) reinterpret_cast<float4*>(albedo)[i] = make_float4(a.x * inv, a.y * inv, a.z * inv, 1.0f);
+    if (normal) reinterpret_cast<float4*>(normal)[i] = make_float4(b.x * inv, b.y * inv, b.z * inv, b.w * inv);
+    if (depth) depth[i] = a.w * inv;
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
+    if (n) k_fill_u32<<<div_up(n, 256), 256, 0, s>>>(p, v, n);
+}
+
+void launch_generate(const FrameParams& f, const PathState& ps, hipStream_t s) {
+    k_generate<<<div_up(f.n_paths, 256), 256, 0, s>>>(f, ps.o, ps.d, ps.T, ps.L, ps.ln0, ps.ln1, ps.q[0],
+                                                      ps.counters);
+}
+
+void launch_shade(const DevScene& S, const FrameParams& f, const PathState& ps, uint32_t bounce, hipStream_t s) {
+    k_shade<<<div_up(f.n_paths, kShadeBlock), kShadeBlock, 0, s>>>(S, f, ps, bounce);
+}
+
+void launch_accumulate(const FrameParams& f, const PathState& ps, float4* accum, float4* moments,
+                       float4* aov_albedo, float4* aov_normal, hipStream_t s) {
+    const uint32_t npix = f.width * f.height;
+    k_accumulate<<<div_up(npix, 256), 256, 0, s>>>(f, ps.L, ps.ln0, ps.ln1, ps.aov0, ps.aov1, accum, moments,
+                                                   aov_albedo, aov_normal);
+}
+
+void launch_resolve(const float4* accum, float4* out, uint32_t n, hipStream_t s) {
+    k_resolve<<<div_up(n, 256), 256, 0, s>>>(accum, out, n);
+}
+
+void launch_tonemap(const float4* accum, uint8_t* out, uint32_t n, hipStream_t s) {
+    k_tonemap<<<div_up(n, 256), 256, 0, s>>>(accum, out, n);
+}
+
+void launch_resolve_aovs(const float4* accum, const float4* aov_albedo, const float4* aov_normal, float* albedo,
+                         float* normal, float* depth, uint32_t n, hipStream_t s) {
+    k_resolve_aovs<<<div_up(n, 256), 256, 0, s>>>(accum, aov_albedo, aov_normal, albedo, normal, depth, n);
+}
+
+}  // namespace ohpt

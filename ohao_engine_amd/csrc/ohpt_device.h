@@ -1,0 +1,484 @@
+// Device-side building blocks of the path-tracing core (gfx950, wave64).
+//
+// Layouts in HBM (all arrays in LBVH leaf order, "slot" = sorted position):
+//   nodes  : (n-1) x 64 B  {c0.lo.xyz, c0.hi.xyz, c1.lo.xyz, c1.hi.xyz, ref0, ref1, parent, pad}
+//            ref bit31 set = leaf (triangle slot), else internal node index
+//   tris   : n x 48 B      {v0.xyz, prim}, {v1.xyz, 0}, {v2.xyz, 0}
+//   attrs  : n x 64 B      {n0.xyz, n1.x}, {n1.yz, n2.xy}, {n2.z, uv0.xy, uv1.x}, {uv1.y, uv2.xy, mat}
+//   mats   : m x 64 B      {base.rgb, rough}, {metal, f0, emis.rg}, {emis.b, albedo_tex, rough_tex, metal_tex}, {0}
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ohpt.h"
+
+#define OHPT_DEV __device__ __forceinline__
+
+namespace ohpt {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+constexpr uint32_t kDimsPerVertex = 9;   // light pick, light uv(2), env uv(2), rr, lobe, dir(2)
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kTwoPi = 6.28318530717958647692f;
+
+// ------------------------------------------------------------------ vec3
+struct f3 { float x, y, z; };
+OHPT_DEV f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+OHPT_DEV f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+OHPT_DEV f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+OHPT_DEV f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+OHPT_DEV f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+OHPT_DEV f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+OHPT_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+OHPT_DEV f3 cross(f3 a, f3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+OHPT_DEV float length(f3 a) { return sqrtf(dot(a, a)); }
+OHPT_DEV f3 normalize(f3 a) { float r = 1.0f / sqrtf(dot(a, a)); return a * r; }
+OHPT_DEV float maxc(f3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+OHPT_DEV float lum(f3 a) { return 0.2126f * a.x + 0.7152f * a.y + 0.0722f * a.z; }
+OHPT_DEV f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+OHPT_DEV float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// ------------------------------------------------------------------ samplers
+// PCG32 (O'Neill 2014): stream = pixel index, state seeded per (sample, vertex).
+struct Pcg32 {
+    uint64_t state, inc;
+    OHPT_DEV uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    OHPT_DEV void seed(uint64_t initstate, uint64_t initseq) {
+        state = 0u;
+        inc = (initseq << 1u) | 1u;
+        next();
+        state += initstate;
+        next();
+    }
+};
+
+OHPT_DEV uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+OHPT_DEV float unit(uint32_t v) { return (float)(v >> 8) * (1.0f / 16777216.0f); }
+
+// Joe-Kuo Sobol direction numbers, dims 0..3 (the reference sampler's table).
+__constant__ static const uint32_t kSobolDirs[128] = {
+    0x80000000u, 0x40000000u, 0x20000000u, 0x10000000u, 0x08000000u, 0x04000000u, 0x02000000u, 0x01000000u,
+    0x00800000u, 0x00400000u, 0x00200000u, 0x00100000u, 0x00080000u, 0x00040000u, 0x00020000u, 0x00010000u,
+    0x00008000u, 0x00004000u, 0x00002000u, 0x00001000u, 0x00000800u, 0x00000400u, 0x00000200u, 0x00000100u,
+    0x00000080u, 0x00000040u, 0x00000020u, 0x00000010u, 0x00000008u, 0x00000004u, 0x00000002u, 0x00000001u,
+    0x80000000u, 0xC0000000u, 0xA0000000u, 0xF0000000u, 0x88000000u, 0xCC000000u, 0xAA000000u, 0xFF000000u,
+    0x80800000u, 0xC0C00000u, 0xA0A00000u, 0xF0F00000u, 0x88880000u, 0xCCCC0000u, 0xAAAA0000u, 0xFFFF0000u,
+    0x80008000u, 0xC000C000u, 0xA000A000u, 0xF000F000u, 0x88008800u, 0xCC00CC00u, 0xAA00AA00u, 0xFF00FF00u,
+    0x80808080u, 0xC0C0C0C0u, 0xA0A0A0A0u, 0xF0F0F0F0u, 0x88888888u, 0xCCCCCCCCu, 0xAAAAAAAAu, 0xFFFFFFFFu,
+    0x80000000u, 0x40000000u, 0xE0000000u, 0x50000000u, 0x98000000u, 0xBC000000u, 0xAE000000u, 0x25000000u,
+    0x5B800000u, 0x91400000u, 0x65600000u, 0x2E500000u, 0x91180000u, 0x88D40000u, 0xDD360000u, 0x02950000u,
+    0x4C888000u, 0x09EB4000u, 0x86CA6000u, 0x13B15000u, 0x42ADD800u, 0x3DF8D400u, 0x2FE57E00u, 0x56EB8100u,
+    0x8C41A680u, 0xE68455C0u, 0x3C9CE8E0u, 0xB44DBFF0u, 0x7C27F6C8u, 0x5239A6ACu, 0x853B0EDEu, 0x52EB2EF9u,
+    0x80000000u, 0xC0000000u, 0x20000000u, 0x50000000u, 0xF8000000u, 0x74000000u, 0xA2000000u, 0x93000000u,
+    0xD8800000u, 0x25400000u, 0x59E00000u, 0xE6D00000u, 0x78080000u, 0xB40C0000u, 0x82020000u, 0xC3050000u,
+    0x208F8000u, 0x51CBC000u, 0xFBEA2000u, 0x75AD5000u, 0xA00AF800u, 0x90077400u, 0xD800A200u, 0x25009300u,
+    0x59E08480u, 0xE6D0CEC0u, 0x78088520u, 0xB40CEBB0u, 0x82028348u, 0xC305C274u, 0x208FB552u, 0x51CB9AE9u,
+};
+
+OHPT_DEV uint32_t sobol_int(uint32_t index, uint32_t dim) {
+    uint32_t r = 0;
+    const uint32_t* d = kSobolDirs + dim * 32u;
+    for (uint32_t bit = 0; index != 0; ++bit, index >>= 1)
+        if (index & 1u) r ^= d[bit];
+    return r;
+}
+
+OHPT_DEV uint32_t owen_scramble(uint32_t v, uint32_t seed) {
+    v = __builtin_bitreverse32(v);
+    v ^= v * 0x3d20adeau;
+    v += seed;
+    v *= (seed >> 16) | 1u;
+    v ^= v * 0x05526c56u;
+    v ^= v * 0x53a22864u;
+    return __builtin_bitreverse32(v);
+}
+
+OHPT_DEV uint32_t hash_pixel(uint32_t x, uint32_t y) {
+    uint32_t h = (x * 0x1b873593u) ^ (y * 0xcc9e2d51u);
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+OHPT_DEV float sobol_owen(uint32_t pixel_seed, uint32_t index, uint32_t dim) {
+    const uint32_t seed = pixel_seed ^ ((dim >> 2) * 0x9e3779b9u);
+    return unit(owen_scramble(sobol_int(index, dim & 3u), seed));
+}
+
+// Draws of one sampler "slot": slot 0 = camera (2 dims), slot 1+b = vertex b (9 dims).
+OHPT_DEV void draw_dims(uint32_t sampler, uint32_t px, uint32_t py, uint32_t width, uint32_t sample_index,
+                        uint32_t slot, uint32_t count, float* out) {
+    if (sampler == OHPT_SAMPLER_SOBOL_OWEN) {
+        const uint32_t ps = hash_pixel(px, py);
+        const uint32_t d0 = slot == 0 ? 0u : 2u + kDimsPerVertex * (slot - 1u);
+        for (uint32_t k = 0; k < count; ++k) out[k] = sobol_owen(ps, sample_index, d0 + k);
+    } else {
+        Pcg32 r;
+        r.seed(splitmix64(((uint64_t)sample_index << 16) | (uint64_t)slot),
+               (uint64_t)py * width + px);
+        for (uint32_t k = 0; k < count; ++k) out[k] = unit(r.next());
+    }
+}
+
+// ------------------------------------------------------------------ scene view
+struct TexDesc {
+    const float* data;     // float4 texels when c >= 2, float when c == 1
+    uint32_t w, h, c, pad;
+};
+
+struct DevScene {
+    const float4* nodes;
+    const float4* tris;
+    const float4* attrs;
+    uint32_t n_tris;
+    uint32_t n_materials;
+    const float4* mats;
+    const TexDesc* tex;
+    uint32_t n_tex;
+    uint32_t n_lights;
+    const float4* lights;          // 5 float4 per light (GPULight)
+    const float4* emit;            // 3 float4 per emitter: {v0, area}, {v1, mat}, {v2, 0}
+    const float* emit_cdf;         // n_emit, strictly increasing, last = 1
+    uint32_t n_emit;
+    float emit_total;              // sum area * lum(Le)
+    const float4* env;             // env_w*env_h RGBA32F, row 0 = +Y
+    const float* env_marg;
+    const float* env_cond;
+    uint32_t env_w, env_h;
+    float env_intensity;
+    uint32_t pad0;
+};
+
+struct Material {
+    f3 albedo;
+    float rough, metal, f0d, alpha;
+};
+
+// ------------------------------------------------------------------ textures
+OHPT_DEV void tex_coords(const TexDesc& t, float u, float v, uint32_t idx[4], float w[4]) {
+    const float x = u * (float)t.w - 0.5f, y = v * (float)t.h - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float fx = x - fx0, fy = y - fy0;
+    int ix = (int)fx0 % (int)t.w; if (ix < 0) ix += (int)t.w;
+    int iy = (int)fy0 % (int)t.h; if (iy < 0) iy += (int)t.h;
+    const int ix1 = (ix + 1 == (int)t.w) ? 0 : ix + 1;
+    const int iy1 = (iy + 1 == (int)t.h) ? 0 : iy + 1;
+    idx[0] = (uint32_t)iy * t.w + (uint32_t)ix;
+    idx[1] = (uint32_t)iy * t.w + (uint32_t)ix1;
+    idx[2] = (uint32_t)iy1 * t.w + (uint32_t)ix;
+    idx[3] = (uint32_t)iy1 * t.w + (uint32_t)ix1;
+    w[0] = (1.0f - fx) * (1.0f - fy);
+    w[1] = fx * (1.0f - fy);
+    w[2] = (1.0f - fx) * fy;
+    w[3] = fx * fy;
+}
+
+OHPT_DEV float4 tex_fetch(const TexDesc& t, uint32_t i) {
+    if (t.c >= 2) return reinterpret_cast<const float4*>(t.data)[i];
+    const float s = t.data[i];
+    return make_float4(s, s, s, s);
+}
+
+OHPT_DEV float4 tex_bilinear(const TexDesc& t, float u, float v) {
+    uint32_t idx[4];
+    float w[4];
+    tex_coords(t, u, v, idx, w);
+    const float4 a = tex_fetch(t, idx[0]), b = tex_fetch(t, idx[1]);
+    const float4 c = tex_fetch(t, idx[2]), d = tex_fetch(t, idx[3]);
+    return make_float4(a.x * w[0] + b.x * w[1] + c.x * w[2] + d.x * w[3],
+                       a.y * w[0] + b.y * w[1] + c.y * w[2] + d.y * w[3],
+                       a.z * w[0] + b.z * w[1] + c.z * w[2] + d.z * w[3],
+                       a.w * w[0] + b.w * w[1] + c.w * w[2] + d.w * w[3]);
+}
+
+OHPT_DEV Material load_material(const DevScene& S, uint32_t m, float u, float v) {
+    m = min(m, S.n_materials - 1u);
+    const float4 m0 = S.mats[m * 4 + 0], m1 = S.mats[m * 4 + 1], m2 = S.mats[m * 4 + 2];
+    Material r;
+    r.albedo = xyz(m0);
+    r.rough = m0.w;
+    r.metal = m1.x;
+    r.f0d = m1.y;
+    const int at = __float_as_int(m2.y), rt = __float_as_int(m2.z), mt = __float_as_int(m2.w);
+    if (at >= 0) { const float4 t = tex_bilinear(S.tex[at], u, v); r.albedo = r.albedo * mk(t.x, t.y, t.z); }
+    if (rt >= 0) r.rough *= tex_bilinear(S.tex[rt], u, v).x;
+    if (mt >= 0) r.metal *= tex_bilinear(S.tex[mt], u, v).x;
+    r.rough = fminf(1.0f, fmaxf(0.04f, r.rough));
+    r.metal = fminf(1.0f, fmaxf(0.0f, r.metal));
+    r.alpha = r.rough * r.rough;
+    return r;
+}
+
+OHPT_DEV f3 material_emission(const DevScene& S, uint32_t m) {
+    m = min(m, S.n_materials - 1u);
+    const float4 m1 = S.mats[m * 4 + 1], m2 = S.mats[m * 4 + 2];
+    return mk(m1.z, m1.w, m2.x);
+}
+
+// ------------------------------------------------------------------ environment
+OHPT_DEV f3 env_eval(const DevScene& S, f3 d) {
+    const float th = acosf(fminf(1.0f, fmaxf(-1.0f, d.y)));
+    const float ph = atan2f(d.z, d.x);
+    const float u = ph * (1.0f / kTwoPi) + 0.5f, v = th * (1.0f / kPi);
+    const float x = u * (float)S.env_w - 0.5f, y = v * (float)S.env_h - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y);
+    const float fx = x - fx0, fy = y - fy0;
+    int ix = (int)fx0 % (int)S.env_w; if (ix < 0) ix += (int)S.env_w;
+    const int ix1 = (ix + 1 == (int)S.env_w) ? 0 : ix + 1;
+    const int iy0 = min(max((int)fy0, 0), (int)S.env_h - 1);
+    const int iy1 = min(max((int)fy0 + 1, 0), (int)S.env_h - 1);
+    const float4 a = S.env[(size_t)iy0 * S.env_w + ix], b = S.env[(size_t)iy0 * S.env_w + ix1];
+    const float4 c = S.env[(size_t)iy1 * S.env_w + ix], e = S.env[(size_t)iy1 * S.env_w + ix1];
+    const float w00 = (1.0f - fx) * (1.0f - fy), w01 = fx * (1.0f - fy), w10 = (1.0f - fx) * fy, w11 = fx * fy;
+    return mk(a.x * w00 + b.x * w01 + c.x * w10 + e.x * w11, a.y * w00 + b.y * w01 + c.y * w10 + e.y * w11,
+              a.z * w00 + b.z * w01 + c.z * w10 + e.z * w11) * S.env_intensity;
+}
+
+// first index with cdf[i] >= u (searchMarginal / searchConditional)
+OHPT_DEV uint32_t cdf_lower_bound(const float* cdf, uint32_t n, float u) {
+    int lo = 0, hi = (int)n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] < u) lo = mid + 1; else hi = mid;
+    }
+    return (uint32_t)lo;
+}
+
+OHPT_DEV bool env_sample(const DevScene& S, float u1, float u2, f3& dir, float& pdf) {
+    const uint32_t W = S.env_w, H = S.env_h;
+    const uint32_t y = cdf_lower_bound(S.env_marg, H, u1);
+    const float* row = S.env_cond + (size_t)y * W;
+    const uint32_t x = cdf_lower_bound(row, W, u2);
+    const float m0 = y > 0 ? S.env_marg[y - 1] : 0.0f, m1 = S.env_marg[y];
+    const float c0 = x > 0 ? row[x - 1] : 0.0f, c1 = row[x];
+    const float dm = m1 - m0, dc = c1 - c0;
+    if (!(dm > 0.0f) || !(dc > 0.0f)) { pdf = 0.0f; return false; }
+    const float fy = fminf(fmaxf((u1 - m0) / dm, 0.0f), 0.99999994f);
+    const float fx = fminf(fmaxf((u2 - c0) / dc, 0.0f), 0.99999994f);
+    const float v = ((float)y + fy) / (float)H, u = ((float)x + fx) / (float)W;
+    const float th = v * kPi, ph = (u - 0.5f) * kTwoPi;
+    float st, ct, sp, cp;
+    sincosf(th, &st, &ct);
+    sincosf(ph, &sp, &cp);
+    dir = mk(st * cp, ct, st * sp);
+    if (!(st > 0.0f)) { pdf = 0.0f; return false; }
+    pdf = dm * dc * (float)W * (float)H / (2.0f * kPi * kPi * st);
+    return pdf > 0.0f;
+}
+
+OHPT_DEV float env_pdf(const DevScene& S, f3 d) {
+    const uint32_t W = S.env_w, H = S.env_h;
+    const float th = acosf(fminf(1.0f, fmaxf(-1.0f, d.y)));
+    const float ph = atan2f(d.z, d.x);
+    const float u = ph * (1.0f / kTwoPi) + 0.5f, v = th * (1.0f / kPi);
+    const int xi = min(max((int)(u * (float)W), 0), (int)W - 1);
+    const int yi = min(max((int)(v * (float)H), 0), (int)H - 1);
+    const float* row = S.env_cond + (size_t)yi * W;
+    const float dc = row[xi] - (xi > 0 ? row[xi - 1] : 0.0f);
+    const float dm = S.env_marg[yi] - (yi > 0 ? S.env_marg[yi - 1] : 0.0f);
+    const float st = sinf(th);
+    if (!(st > 0.0f)) return 0.0f;
+    return fmaxf(0.0f, dm * dc * (float)W * (float)H / (2.0f * kPi * kPi * st));
+}
+
+// ------------------------------------------------------------------ BSDF
+struct Frame {
+    f3 t, b, n;
+};
+// Duff et al. 2017, "Building an orthonormal basis, revisited".
+OHPT_DEV Frame make_frame(f3 n) {
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float bb = n.x * n.y * a;
+    Frame f;
+    f.n = n;
+    f.t = mk(1.0f + sign * n.x * n.x * a, sign * bb, -sign * n.x);
+    f.b = mk(bb, sign + n.y * n.y * a, -n.y);
+    return f;
+}
+OHPT_DEV f3 to_local(const Frame& f, f3 v) { return mk(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
+OHPT_DEV f3 to_world(const Frame& f, f3 v) { return f.t * v.x + f.b * v.y + f.n * v.z; }
+
+OHPT_DEV float smith_lambda(float c, float a) {
+    const float c2 = c * c;
+    const float t2 = fmaxf(0.0f, 1.0f - c2) / c2;
+    return 0.5f * (-1.0f + sqrtf(1.0f + a * a * t2));
+}
+OHPT_DEV float ggx_d(float ch, float a) {
+    const float a2 = a * a;
+    const float den = ch * ch * (a2 - 1.0f) + 1.0f;
+    return a2 / (kPi * den * den);
+}
+// Lobe-selection probability: depends on metallic and view angle only, so the
+// sampling decisions do not move with albedo.
+OHPT_DEV float lobe_prob(const Material& m, float cos_o) {
+    float fd = 0.0f;
+    if (m.f0d > 0.0f) {
+        const float q = 1.0f - cos_o;
+        const float q5 = q * q * q * q * q;
+        fd = fminf(1.0f, fmaxf(0.25f, m.f0d + (1.0f - m.f0d) * q5));
+    }
+    return m.metal + (1.0f - m.metal) * fd;
+}
+
+struct BsdfTerms {           // pieces reused by the adjoint
+    float D, G2, s, co, ci;
+};
+
+OHPT_DEV f3 bsdf_eval(const Material& m, f3 ns, f3 wo, f3 wi, BsdfTerms* terms = nullptr) {
+    const float co = dot(ns, wo), ci = dot(ns, wi);
+    if (!(co > 0.0f) || !(ci > 0.0f)) return mk(0, 0, 0);
+    const f3 h = normalize(wo + wi);
+    const float ch = fmaxf(0.0f, dot(ns, h));
+    const float voh = fminf(1.0f, fmaxf(0.0f, dot(wo, h)));
+    const float D = ggx_d(ch, m.alpha);
+    const float G2 = 1.0f / (1.0f + smith_lambda(co, m.alpha) + smith_lambda(ci, m.alpha));
+    const float q = 1.0f - voh;
+    const float s = q * q * q * q * q;
+    const f3 F0 = mk(m.f0d + (m.albedo.x - m.f0d) * m.metal, m.f0d + (m.albedo.y - m.f0d) * m.metal,
+                     m.f0d + (m.albedo.z - m.f0d) * m.metal);
+    // Schlick with F90 = 1 for any specular surface, 0 for a pure Lambert one
+    const float F90 = (m.f0d > 0.0f || m.metal > 0.0f) ? 1.0f : 0.0f;
+    const f3 F = mk(F0.x + (F90 - F0.x) * s, F0.y + (F90 - F0.y) * s, F0.z + (F90 - F0.z) * s);
+    const float sp = D * G2 / (4.0f * co * ci);
+    const float kd = (1.0f - m.metal) * (1.0f / kPi);
+    if (terms) { terms->D = D; terms->G2 = G2; terms->s = s; terms->co = co; terms->ci = ci; }
+    return mk(F.x * sp + (1.0f - F.x) * kd * m.albedo.x, F.y * sp + (1.0f - F.y) * kd * m.albedo.y,
+              F.z * sp + (1.0f - F.z) * kd * m.albedo.z);
+}
+
+OHPT_DEV float bsdf_pdf(const Material& m, f3 ns, f3 wo, f3 wi) {
+    const float co = dot(ns, wo), ci = dot(ns, wi);
+    if (!(co > 0.0f) || !(ci > 0.0f)) return 0.0f;
+    const float ps = lobe_prob(m, co);
+    float pspec = 0.0f;
+    if (ps > 0.0f) {
+        const f3 h = normalize(wo + wi);
+        const float ch = fmaxf(0.0f, dot(ns, h));
+        const float G1 = 1.0f / (1.0f + smith_lambda(co, m.alpha));
+        pspec = G1 * ggx_d(ch, m.alpha) / (4.0f * co);
+    }
+    return ps * pspec + (1.0f - ps) * ci * (1.0f / kPi);
+}
+
+// Heitz 2018, "Sampling the GGX Distribution of Visible Normals" (isotropic).
+OHPT_DEV f3 sample_vndf(f3 Ve, float a, float u1, float u2) {
+    const f3 Vh = normalize(mk(a * Ve.x, a * Ve.y, Ve.z));
+    const float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+    const f3 T1 = lensq > 0.0f ? mk(-Vh.y, Vh.x, 0.0f) * (1.0f / sqrtf(lensq)) : mk(1, 0, 0);
+    const f3 T2 = cross(Vh, T1);
+    const float r = sqrtf(u1);
+    float sp, cp;
+    sincosf(kTwoPi * u2, &sp, &cp);
+    const float t1 = r * cp;
+    float t2 = r * sp;
+    const float s = 0.5f * (1.0f + Vh.z);
+    t2 = (1.0f - s) * sqrtf(fmaxf(0.0f, 1.0f - t1 * t1)) + s * t2;
+    const f3 Nh = T1 * t1 + T2 * t2 + Vh * sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2));
+    return normalize(mk(a * Nh.x, a * Nh.y, fmaxf(0.0f, Nh.z)));
+}
+
+OHPT_DEV bool bsdf_sample(const Material& m, f3 ns, f3 wo, float ul, float u1, float u2, f3& wi) {
+    const Frame fr = make_frame(ns);
+    const f3 wl = to_local(fr, wo);
+    if (!(wl.z > 0.0f)) return false;
+    const float ps = lobe_prob(m, wl.z);
+    f3 il;
+    if (ul < ps) {
+        const f3 hl = sample_vndf(wl, m.alpha, u1, u2);
+        il = hl * (2.0f * dot(wl, hl)) - wl;
+    } else {
+        const float r = sqrtf(u1);
+        float sp, cp;
+        sincosf(kTwoPi * u2, &sp, &cp);
+        il = mk(r * cp, r * sp, sqrtf(fmaxf(0.0f, 1.0f - u1)));
+    }
+    if (!(il.z > 0.0f)) return false;
+    wi = normalize(to_world(fr, il));
+    return true;
+}
+
+OHPT_DEV float balance(float a, float b) { return a / fmaxf(a + b, 1e-30f); }
+
+// ------------------------------------------------------------------ analytic lights
+// Reference light formulas, pt_raygen_offline.rgen:311-389 (Le = color*I/max(4*3.14159*r^2, 0.01)).
+OHPT_DEV bool sample_analytic(const float4* L5, f3 p, float u1, float u2, f3& wi, float& dist, f3& Lw) {
+    const float4 pt = L5[0], ci = L5[1], dp = L5[2], ex = L5[3], ex2 = L5[4];
+    const float type = pt.w;
+    const f3 c = xyz(pt);
+    const float rad = fmaxf(dp.w, 0.01f);
+    const float area4 = 4.0f * 3.14159f * rad * rad;
+    const f3 Le = xyz(ci) * (ci.w / fmaxf(area4, 0.01f));
+    float weight = 0.0f;
+    if (type < 0.5f || (type >= 1.5f && type < 2.5f)) {
+        const float ct = 1.0f - 2.0f * u1;
+        const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        float sp, cp;
+        sincosf(6.2831853f * u2, &sp, &cp);
+        const f3 off = mk(st * cp, st * sp, ct) * rad;
+        const f3 toL = (c + off) - p;
+        dist = length(toL);
+        if (!(dist > 0.0f)) return false;
+        wi = toL * (1.0f / dist);
+        if (type < 0.5f) {
+            const float lc = fmaxf(dot(-wi, normalize(off)), 0.0f);
+            weight = lc * area4 / (dist * dist);
+        } else {
+            const float ca = dot(-wi, normalize(xyz(dp)));
+            const float ic = cosf(dp.w * (kPi / 180.0f));
+            const float oc = cosf(ex.w * (kPi / 180.0f));
+            float sf = fminf(1.0f, fmaxf(0.0f, (ca - oc) / (ic - oc + 0.001f)));
+            sf *= sf;
+            weight = area4 / (dist * dist) * sf;
+        }
+    } else if (type < 1.5f) {
+        wi = -normalize(xyz(dp));
+        dist = INFINITY;
+        weight = 1.0f;
+    } else {
+        const f3 e1 = xyz(ex), e2 = xyz(ex2);
+        const f3 toL = (c + e1 * u1 + e2 * u2) - p;
+        dist = length(toL);
+        if (!(dist > 0.0f)) return false;
+        wi = toL * (1.0f / dist);
+        const float lc = fmaxf(dot(-wi, normalize(cross(e1, e2))), 0.0f);
+        weight = lc * ex2.w / (dist * dist);
+    }
+    Lw = Le * weight;
+    return weight > 0.0f;
+}
+
+// ------------------------------------------------------------------ wave helpers
+OHPT_DEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Wave-aggregated append: returns the slot for this lane (valid only when pred).
+OHPT_DEV uint32_t wave_append(uint32_t* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const uint32_t lo = (uint32_t)mask, hi = (uint32_t)(mask >> 32);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + rank;
+}
+
+}  // namespace ohpt

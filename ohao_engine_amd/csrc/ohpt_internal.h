@@ -1,0 +1,102 @@
+// Host-side declarations shared by the translation units of libohpt.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ohpt_device.h"
+
+namespace ohpt {
+
+// ---- LBVH build (lbvh.hip) --------------------------------------------------
+struct LbvhBuffers {
+    // inputs (device): tri_pos n*9 floats (global order), nrm n*9 (or null), uv n*6 (or null), mat n
+    const float* tri_pos;
+    const float* tri_nrm;
+    const float* tri_uv;
+    const uint32_t* tri_mat;
+    uint32_t n;
+    // outputs (device, preallocated): nodes (max(n-1,1))*64 B, tris n*48 B, attrs n*64 B, prim n
+    float4* nodes;
+    float4* tris;
+    float4* attrs;
+    uint32_t* prim;        // slot -> original triangle id
+    uint32_t* slot_of;     // original triangle id -> slot
+    // scratch (device): see lbvh_scratch_bytes
+    void* scratch;
+};
+size_t lbvh_scratch_bytes(uint32_t n);
+// Returns 0 or a HIP error; writes max leaf depth to *depth_out (host) when not null.
+hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out);
+
+// Standalone stable radix sort of 32-bit keys with 32-bit values (exposed for tests).
+size_t radix_sort_scratch_bytes(uint32_t n);
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, void* scratch,
+                            hipStream_t s);
+
+// ---- traversal (traverse.hip) ---------------------------------------------
+constexpr uint32_t kTraceBlock = 128;
+constexpr uint32_t kStackLds = 32;       // per-thread LDS stack entries
+constexpr uint32_t kStackOvf = 32;       // per-ray global overflow entries
+constexpr uint32_t kMaxTreeDepth = 60;   // register(4) + LDS + overflow comfortably above this
+
+// Closest hit for queued paths: ray of path queue[j] -> hits[j] = (t, slot|miss, u, v).
+void launch_extend(const DevScene& S, const uint32_t* queue, const uint32_t* count, uint32_t max_count,
+                   const float4* ray_o, const float4* ray_d, float4* hits, uint32_t* ovf,
+                   unsigned long long* stats, hipStream_t s);
+// Shadow rays: sh_o[j] = (o, tmax), sh_d[j] = (d, pid*2+k bits), sh_c[j] = contribution;
+// unoccluded ones add sh_c into ln[k][pid].
+void launch_connect(const DevScene& S, const uint32_t* count, uint32_t max_count, const float4* sh_o,
+                    const float4* sh_d, const float4* sh_c, float4* ln0, float4* ln1, uint32_t* ovf,
+                    hipStream_t s);
+// Caller rays (ox,oy,oz,tmin,dx,dy,dz,tmax) -> (t, prim|miss, u, v) / occluded flags.
+void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* ovf,
+                       hipStream_t s);
+void launch_occluded_rays(const DevScene& S, const float4* rays, uint32_t* out, uint32_t n, uint32_t* ovf,
+                          hipStream_t s);
+
+// ---- integrator (integrator.hip) ------------------------------------------
+struct FrameParams {
+    ohpt_camera cam;
+    uint32_t width, height;
+    uint32_t n_paths;          // width*height*spp_batch
+    uint32_t sample_base;      // sample index of the first sample of this batch
+    uint32_t max_bounces, rr_start, sampler, flags;
+    float firefly_clamp, ray_eps;
+};
+
+struct PathState {
+    float4* o;        // ray origin
+    float4* d;        // ray dir, w = previous BSDF pdf
+    float4* T;        // throughput
+    float4* L;        // radiance from emission / env hits
+    float4* ln0;      // light NEE radiance
+    float4* ln1;      // env NEE radiance
+    float4* aov0;     // first-hit albedo.xyz, depth
+    float4* aov1;     // first-hit normal.xyz, hit flag
+    uint32_t* q[2];   // extend queues (ping-pong)
+    float4* hits;
+    float4* sh_o;
+    float4* sh_d;
+    float4* sh_c;
+    uint32_t* counters;   // [0..kMaxBounce+1]: extend counts; [64..]: shadow counts
+    uint32_t capacity;
+};
+constexpr uint32_t kShadowCounterBase = 64;
+
+void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
+void launch_generate(const FrameParams& f, const PathState& ps, hipStream_t s);
+void launch_shade(const DevScene& S, const FrameParams& f, const PathState& ps, uint32_t bounce, hipStream_t s);
+void launch_accumulate(const FrameParams& f, const PathState& ps, float4* accum, float4* moments,
+                       float4* aov_albedo, float4* aov_normal, hipStream_t s);
+void launch_resolve(const float4* accum, float4* out, uint32_t n, hipStream_t s);
+void launch_tonemap(const float4* accum, uint8_t* out, uint32_t n, hipStream_t s);
+void launch_resolve_aovs(const float4* accum, const float4* aov_albedo, const float4* aov_normal,
+                         float* albedo, float* normal, float* depth, uint32_t n, hipStream_t s);
+
+// ---- environment (env.hip) -------------------------------------------------
+// Per-row luminance*sin(theta) prefix sums, normalised, plus marginal (env_cdf.cpp:13-61).
+hipError_t env_cdf_build(const float4* env, uint32_t W, uint32_t H, float* marg, float* cond,
+                         float* row_sum, float* integral_dev, hipStream_t s);
+
+}  // namespace ohpt

@@ -1,0 +1,301 @@
+// Extend (closest hit) and connect (any hit) kernels.
+//
+// Short-stack traversal of the 64-byte two-child LBVH nodes: the top four
+// stack entries live in VGPRs (shifted on push/pop), deeper entries spill to a
+// per-thread column of LDS ([depth][thread], bank-conflict free), and beyond
+// that to a per-ray slot in global memory. Leaves (one triangle each) are
+// intersected as soon as their box is hit, near child first; the visiting
+// order and every FP operation match the sequential restatement in the
+// oracle, so hits agree bit for bit.
+//
+// Ray/box: robust slab test with tmax scaled by 1+2*gamma(3) (Ize 2013).
+// Ray/triangle: watertight test of Woop, Benthin & Wald (JCGT 2013) with the
+// double-precision fallback on zero edge functions.
+#pragma clang fp contract(off)
+
+#include "ohpt_internal.h"
+
+namespace ohpt {
+namespace {
+
+constexpr float kGamma3Scale = 1.0f + 2.0f * (3.0f * 5.9604645e-08f) / (1.0f - 3.0f * 5.9604645e-08f);
+
+struct RayPre {
+    float ox, oy, oz;
+    float ix, iy, iz;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ float sel(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
+
+__device__ __forceinline__ RayPre ray_setup(float3 o, float3 d) {
+    RayPre r;
+    float dx = d.x, dy = d.y, dz = d.z;
+    if (fabsf(dx) < 1e-20f) dx = copysignf(1e-20f, dx);
+    if (fabsf(dy) < 1e-20f) dy = copysignf(1e-20f, dy);
+    if (fabsf(dz) < 1e-20f) dz = copysignf(1e-20f, dz);
+    r.ox = o.x; r.oy = o.y; r.oz = o.z;
+    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    const int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    const float dkz = sel(dx, dy, dz, kz);
+    if (dkz < 0.0f) { const int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = sel(dx, dy, dz, kx) / dkz;
+    r.Sy = sel(dx, dy, dz, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    return r;
+}
+
+// entry distance, or +inf when the box is missed
+__device__ __forceinline__ float box_hit(const RayPre& r, float lx, float ly, float lz, float hx, float hy,
+                                         float hz, float tmin, float tmax) {
+    const float ax = (lx - r.ox) * r.ix, bx = (hx - r.ox) * r.ix;
+    const float ay = (ly - r.oy) * r.iy, by = (hy - r.oy) * r.iy;
+    const float az = (lz - r.oz) * r.iz, bz = (hz - r.oz) * r.iz;
+    float t0 = tmin, t1 = tmax;
+    t0 = fmaxf(t0, fminf(ax, bx)); t1 = fminf(t1, fmaxf(ax, bx) * kGamma3Scale);
+    t0 = fmaxf(t0, fminf(ay, by)); t1 = fminf(t1, fmaxf(ay, by) * kGamma3Scale);
+    t0 = fmaxf(t0, fminf(az, bz)); t1 = fminf(t1, fmaxf(az, bz) * kGamma3Scale);
+    return (t0 <= t1) ? t0 : INFINITY;
+}
+
+__device__ __forceinline__ bool tri_hit(const RayPre& r, float4 a, float4 b, float4 c, float tmin, float tmax,
+                                        float& tout, float& uout, float& vout) {
+    const float Ax0 = a.x - r.ox, Ay0 = a.y - r.oy, Az0 = a.z - r.oz;
+    const float Bx0 = b.x - r.ox, By0 = b.y - r.oy, Bz0 = b.z - r.oz;
+    const float Cx0 = c.x - r.ox, Cy0 = c.y - r.oy, Cz0 = c.z - r.oz;
+    const float Akz = sel(Ax0, Ay0, Az0, r.kz), Bkz = sel(Bx0, By0, Bz0, r.kz), Ckz = sel(Cx0, Cy0, Cz0, r.kz);
+    const float Ax = sel(Ax0, Ay0, Az0, r.kx) - r.Sx * Akz;
+    const float Ay = sel(Ax0, Ay0, Az0, r.ky) - r.Sy * Akz;
+    const float Bx = sel(Bx0, By0, Bz0, r.kx) - r.Sx * Bkz;
+    const float By = sel(Bx0, By0, Bz0, r.ky) - r.Sy * Bkz;
+    const float Cx = sel(Cx0, Cy0, Cz0, r.kx) - r.Sx * Ckz;
+    const float Cy = sel(Cx0, Cy0, Cz0, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    if (det < 0.0f) { T = -T; det = -det; U = -U; V = -V; W = -W; }
+    if (!(T > tmin * det) || !(T < tmax * det)) return false;
+    const float rcp = 1.0f / det;
+    tout = T * rcp;
+    uout = V * rcp;
+    vout = W * rcp;
+    return true;
+}
+
+// Register/LDS/global short stack.
+struct Stack {
+    uint32_t r0, r1, r2, r3;
+    int nreg;     // entries held in registers (top of stack)
+    int nmem;     // entries below them (LDS, then overflow)
+    uint32_t* lds;     // column base: lds[k * kTraceBlock]
+    uint32_t* ovf;     // per-ray overflow base
+    __device__ __forceinline__ void push(uint32_t x) {
+        if (nreg == 4) {
+            if (nmem < (int)kStackLds) lds[nmem * kTraceBlock] = r3;
+            else ovf[nmem - (int)kStackLds] = r3;
+            ++nmem;
+            --nreg;
+        }
+        r3 = r2; r2 = r1; r1 = r0; r0 = x;
+        ++nreg;
+    }
+    __device__ __forceinline__ bool empty() const { return nreg == 0 && nmem == 0; }
+    __device__ __forceinline__ uint32_t pop() {
+        if (nreg > 0) {
+            const uint32_t x = r0;
+            r0 = r1; r1 = r2; r2 = r3;
+            --nreg;
+            return x;
+        }
+        --nmem;
+        return nmem < (int)kStackLds ? lds[nmem * kTraceBlock] : ovf[nmem - (int)kStackLds];
+    }
+};
+
+template <bool kAnyHit>
+__device__ __forceinline__ float4 traverse(const DevScene& S, float3 o, float3 d, float tmin, float tmax,
+                                           uint32_t* lds_col, uint32_t* ovf, uint32_t& visits,
+                                           uint32_t& tests) {
+    float4 hit = make_float4(tmax, __uint_as_float(kMiss), 0.0f, 0.0f);
+    if (S.n_tris == 0) return hit;
+    const RayPre r = ray_setup(o, d);
+    Stack st;
+    st.r0 = st.r1 = st.r2 = st.r3 = 0;
+    st.nreg = 0; st.nmem = 0;
+    st.lds = lds_col; st.ovf = ovf;
+    uint32_t cur = 0;
+    float tclose = tmax;
+    uint32_t nv = 0, nt = 0;
+    for (;;) {
+        ++nv;
+        const float4* np = S.nodes + (size_t)cur * 4;
+        const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+        const float e0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tmin, tclose);
+        const float e1 = box_hit(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tmin, tclose);
+        const uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
+        const bool h0 = e0 != INFINITY, h1 = e1 != INFINITY;
+        const bool sw = h1 && (!h0 || e1 < e0);
+        const uint32_t A = sw ? c1 : c0, B = sw ? c0 : c1;
+        const bool hA = sw ? h1 : h0, hB = sw ? h0 : h1;
+        const bool lA = hA && (A & kLeafBit), lB = hB && (B & kLeafBit);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const bool test = k == 0 ? lA : lB;
+            if (test) {
+                const uint32_t s = (k == 0 ? A : B) & ~kLeafBit;
+                const float4* tp = S.tris + (size_t)s * 3;
+                const float4 ta = tp[0], tb = tp[1], tc = tp[2];
+                float tt, uu, vv;
+                ++nt;
+                if (tri_hit(r, ta, tb, tc, tmin, tclose, tt, uu, vv)) {
+                    tclose = tt;
+                    hit = make_float4(tt, __uint_as_float(s), uu, vv);
+                    if (kAnyHit) {
+                        visits = nv;
+                        tests = nt;
+                        return hit;
+                    }
+                }
+            }
+        }
+        const bool iA = hA && !(A & kLeafBit), iB = hB && !(B & kLeafBit);
+        if (iA) {
+            cur = A;
+            if (iB) st.push(B);
+        } else if (iB) {
+            cur = B;
+        } else {
+            if (st.empty()) break;
+            cur = st.pop();
+        }
+    }
+    visits = nv;
+    tests = nt;
+    return hit;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_extend(DevScene S, const uint32_t* __restrict__ queue,
+                                                        const uint32_t* __restrict__ count,
+                                                        const float4* __restrict__ ray_o,
+                                                        const float4* __restrict__ ray_d,
+                                                        float4* __restrict__ hits, uint32_t* __restrict__ ovf,
+                                                        unsigned long long* __restrict__ stats) {
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
+    const bool active = j < *count;
+    uint32_t nv = 0, nt = 0;
+    if (active) {
+        const uint32_t pid = queue[j];
+        const float4 o = ray_o[pid], d = ray_d[pid];
+        hits[j] = traverse<false>(S, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), 0.0f, INFINITY,
+                                  lds + threadIdx.x, ovf + (size_t)j * kStackOvf, nv, nt);
+    }
+    if (stats) {
+        // node visits and triangle tests of this launch: wave sums, one atomic per wave
+        for (int off = 32; off > 0; off >>= 1) {
+            nv += __shfl_xor(nv, off);
+            nt += __shfl_xor(nt, off);
+        }
+        if ((threadIdx.x & 63u) == 0) {
+            atomicAdd(&stats[0], (unsigned long long)nv);
+            atomicAdd(&stats[1], (unsigned long long)nt);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_connect(DevScene S, const uint32_t* __restrict__ count,
+                                                         const float4* __restrict__ sh_o,
+                                                         const float4* __restrict__ sh_d,
+                                                         const float4* __restrict__ sh_c, float4* __restrict__ ln0,
+                                                         float4* __restrict__ ln1, uint32_t* __restrict__ ovf) {
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (j >= *count) return;
+    const float4 o = sh_o[j], d = sh_d[j];
+    uint32_t nv = 0, nt = 0;
+    const float4 h = traverse<true>(S, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), 0.0f, o.w,
+                                    lds + threadIdx.x, ovf + (size_t)j * kStackOvf, nv, nt);
+    if (__float_as_uint(h.y) != kMiss) return;
+    const uint32_t tag = __float_as_uint(d.w);
+    const uint32_t pid = tag >> 1;
+    float4* dst = (tag & 1u) ? ln1 : ln0;
+    const float4 c = sh_c[j];
+    float4 v = dst[pid];
+    v.x += c.x; v.y += c.y; v.z += c.z;
+    dst[pid] = v;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DevScene S, const float4* __restrict__ rays,
+                                                            float4* __restrict__ hits, uint32_t n,
+                                                            uint32_t* __restrict__ ovf) {
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (j >= n) return;
+    const float4 o = rays[(size_t)j * 2], d = rays[(size_t)j * 2 + 1];
+    uint32_t nv = 0, nt = 0;
+    float4 h = traverse<false>(S, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), o.w, d.w,
+                               lds + threadIdx.x, ovf + (size_t)j * kStackOvf, nv, nt);
+    const uint32_t s = __float_as_uint(h.y);
+    if (s != kMiss) h.y = S.tris[(size_t)s * 3].w;   // slot -> caller's triangle id
+    hits[j] = h;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_occluded_rays(DevScene S, const float4* __restrict__ rays,
+                                                               uint32_t* __restrict__ out, uint32_t n,
+                                                               uint32_t* __restrict__ ovf) {
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (j >= n) return;
+    const float4 o = rays[(size_t)j * 2], d = rays[(size_t)j * 2 + 1];
+    uint32_t nv = 0, nt = 0;
+    const float4 h = traverse<true>(S, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), o.w, d.w,
+                                    lds + threadIdx.x, ovf + (size_t)j * kStackOvf, nv, nt);
+    out[j] = __float_as_uint(h.y) != kMiss ? 1u : 0u;
+}
+
+inline uint32_t blocks_for(uint32_t n) { return (n + kTraceBlock - 1) / kTraceBlock; }
+
+}  // namespace
+
+void launch_extend(const DevScene& S, const uint32_t* queue, const uint32_t* count, uint32_t max_count,
+                   const float4* ray_o, const float4* ray_d, float4* hits, uint32_t* ovf,
+                   unsigned long long* stats, hipStream_t s) {
+    if (max_count == 0) return;
+    k_extend<<<blocks_for(max_count), kTraceBlock, 0, s>>>(S, queue, count, ray_o, ray_d, hits, ovf, stats);
+}
+
+void launch_connect(const DevScene& S, const uint32_t* count, uint32_t max_count, const float4* sh_o,
+                    const float4* sh_d, const float4* sh_c, float4* ln0, float4* ln1, uint32_t* ovf,
+                    hipStream_t s) {
+    if (max_count == 0) return;
+    k_connect<<<blocks_for(max_count), kTraceBlock, 0, s>>>(S, count, sh_o, sh_d, sh_c, ln0, ln1, ovf);
+}
+
+void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* ovf,
+                       hipStream_t s) {
+    if (n == 0) return;
+    k_trace_rays<<<blocks_for(n), kTraceBlock, 0, s>>>(S, rays, hits, n, ovf);
+}
+
+void launch_occluded_rays(const DevScene& S, const float4* rays, uint32_t* out, uint32_t n, uint32_t* ovf,
+                          hipStream_t s) {
+    if (n == 0) return;
+    k_occluded_rays<<<blocks_for(n), kTraceBlock, 0, s>>>(S, rays, out, n, ovf);
+}
+
+}  // namespace ohpt

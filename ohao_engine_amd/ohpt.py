@@ -1,0 +1,327 @@
+"""ctypes binding of libohpt.so (include/ohpt.h) and a Python mirror of the
+reference renderer façade used by the three example drivers
+(VulkanRenderer, ohao/gpu/vulkan/renderer.hpp:114-292).
+
+The library must be built (python -m ohao_engine_amd.build); loading fails
+loudly when it is missing — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from . import scenes as S
+
+LIB_PATH = Path(__file__).resolve().parent / "libohpt.so"
+
+OK = 0
+RENDER_ASYNC = 1
+RENDER_PROFILE = 2
+
+
+class OhptError(RuntimeError):
+    pass
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("forward", C.c_float * 3), ("right", C.c_float * 3),
+                ("up", C.c_float * 3), ("tan_half_fov_y", C.c_float)]
+
+
+class Settings(C.Structure):
+    _fields_ = [("max_bounces", C.c_uint32), ("rr_start", C.c_uint32), ("sampler", C.c_uint32),
+                ("flags", C.c_uint32), ("firefly_clamp", C.c_float), ("ray_eps", C.c_float),
+                ("reserved", C.c_uint32 * 2)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("ms_extend", C.c_double), ("ms_shade", C.c_double),
+                ("ms_connect", C.c_double), ("ms_build", C.c_double), ("rays_extend", C.c_uint64),
+                ("rays_shadow", C.c_uint64), ("paths", C.c_uint64), ("node_visits", C.c_uint64),
+                ("tri_tests", C.c_uint64), ("extend_launches", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+# (name, restype, argtypes) of every entry point declared in include/ohpt.h
+_vp, _f32p, _u32p, _u8p = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)
+SIGNATURES = [
+    ("ohpt_create", C.c_int, [C.POINTER(_vp), C.c_int, C.c_uint32, C.c_uint32]),
+    ("ohpt_destroy", C.c_int, [_vp]),
+    ("ohpt_resize", C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    ("ohpt_last_error", C.c_char_p, []),
+    ("ohpt_abi_version", C.c_int, []),
+    ("ohpt_scene_begin", C.c_int, [_vp]),
+    ("ohpt_scene_add_mesh", C.c_int, [_vp, _f32p, _f32p, _f32p, C.c_uint32, _u32p, C.c_uint32, _u32p,
+                                      C.c_uint32, _f32p]),
+    ("ohpt_scene_commit", C.c_int, [_vp]),
+    ("ohpt_scene_triangle_count", C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    ("ohpt_scene_tree_depth", C.c_int, [_vp, _u32p]),
+    ("ohpt_scene_export_bvh", C.c_int, [_vp, _vp, _vp, _u32p]),
+    ("ohpt_upload_materials", C.c_int, [_vp, _vp, C.c_uint32]),
+    ("ohpt_upload_texture", C.c_int, [_vp, C.POINTER(C.c_int32), _f32p, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("ohpt_update_texture", C.c_int, [_vp, C.c_int32, _f32p, C.c_int]),
+    ("ohpt_texture_info", C.c_int, [_vp, C.c_int32, _u32p, _u32p, _u32p]),
+    ("ohpt_upload_lights", C.c_int, [_vp, _vp, C.c_uint32]),
+    ("ohpt_upload_env", C.c_int, [_vp, _f32p, C.c_uint32, C.c_uint32]),
+    ("ohpt_set_env_intensity", C.c_int, [_vp, C.c_float]),
+    ("ohpt_export_env_cdf", C.c_int, [_vp, _f32p, _f32p, _f32p]),
+    ("ohpt_set_camera", C.c_int, [_vp, C.POINTER(Camera)]),
+    ("ohpt_set_settings", C.c_int, [_vp, C.POINTER(Settings)]),
+    ("ohpt_get_settings", C.c_int, [_vp, C.POINTER(Settings)]),
+    ("ohpt_reset_accumulation", C.c_int, [_vp]),
+    ("ohpt_set_render_seed", C.c_int, [_vp, C.c_uint32]),
+    ("ohpt_render", C.c_int, [_vp, C.c_uint32, C.c_uint32]),
+    ("ohpt_synchronize", C.c_int, [_vp]),
+    ("ohpt_get_stats", C.c_int, [_vp, C.POINTER(Stats)]),
+    ("ohpt_read_accum", C.c_int, [_vp, _f32p, C.c_int]),
+    ("ohpt_read_moments", C.c_int, [_vp, _f32p, _f32p, C.c_int]),
+    ("ohpt_read_pixels", C.c_int, [_vp, _u8p, C.c_int]),
+    ("ohpt_read_aovs", C.c_int, [_vp, _f32p, _f32p, _f32p, C.c_int]),
+    ("ohpt_trace_rays", C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_int]),
+    ("ohpt_occluded_rays", C.c_int, [_vp, _f32p, _u32p, C.c_uint32, C.c_int]),
+]
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libohpt.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise OhptError(f"{LIB_PATH} is missing: build it with `python -m ohao_engine_amd.build` "
+                            "(the path tracer has no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code: int, what: str):
+    if code != OK:
+        msg = lib().ohpt_last_error().decode(errors="replace")
+        raise OhptError(f"{what} failed ({code}): {msg}")
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_f32p)
+
+
+def _dev_ptr(t):
+    """Raw device pointer of a torch tensor (zero-copy interop)."""
+    return C.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One ohpt context (one HIP stream on one device)."""
+
+    def __init__(self, width: int, height: int, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        _check(self.L.ohpt_create(C.byref(h), device, width, height), "ohpt_create")
+        self.h = h
+        self.width, self.height = width, height
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ohpt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- scene ----------------------------------------------------------------
+    def load_scene(self, sc: "S.Scene", upload_geometry: bool = True):
+        if upload_geometry:
+            self.set_geometry(sc.pos, sc.nrm, sc.uv, sc.mat)
+        tex_ids = [self.upload_texture(t) for t in sc.textures]
+        assert tex_ids == list(range(len(sc.textures)))
+        self.set_materials(sc.materials)
+        self.set_lights(sc.lights)
+        self.set_env(sc.env)
+        self.set_env_intensity(sc.env_intensity)
+        if sc.camera is not None:
+            self.set_camera(sc.camera)
+        self.set_settings(sc.settings)
+        self.set_render_seed(sc.settings.seed)
+        self.reset()
+
+    def set_geometry(self, pos, nrm=None, uv=None, mat=None):
+        """Flattened world-space triangles (n, 9) as one mesh (global ids = row order)."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 9)
+        n = pos.shape[0]
+        verts = pos.reshape(-1, 3)
+        nv = None if nrm is None else np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        uvv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        idx = np.arange(3 * n, dtype=np.uint32)
+        m = None if mat is None else np.ascontiguousarray(mat, np.uint32)
+        _check(self.L.ohpt_scene_begin(self.h), "ohpt_scene_begin")
+        if n:
+            _check(self.L.ohpt_scene_add_mesh(self.h, _fp(verts), _fp(nv), _fp(uvv), 3 * n,
+                                              idx.ctypes.data_as(_u32p), n,
+                                              None if m is None else m.ctypes.data_as(_u32p), 0, None),
+                   "ohpt_scene_add_mesh")
+        _check(self.L.ohpt_scene_commit(self.h), "ohpt_scene_commit")
+
+    def add_mesh(self, positions, indices, normals=None, uvs=None, mat_ids=None, material_base=0, world=None):
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        n = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        u = None if uvs is None else np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+        m = None if mat_ids is None else np.ascontiguousarray(mat_ids, np.uint32)
+        w = None if world is None else np.ascontiguousarray(np.asarray(world, np.float32).T.reshape(-1))
+        _check(self.L.ohpt_scene_add_mesh(self.h, _fp(p), _fp(n), _fp(u), p.shape[0], i.ctypes.data_as(_u32p),
+                                          i.size // 3, None if m is None else m.ctypes.data_as(_u32p),
+                                          material_base, _fp(w)), "ohpt_scene_add_mesh")
+
+    def begin_scene(self):
+        _check(self.L.ohpt_scene_begin(self.h), "ohpt_scene_begin")
+
+    def commit_scene(self):
+        _check(self.L.ohpt_scene_commit(self.h), "ohpt_scene_commit")
+
+    def tree_depth(self) -> int:
+        d = C.c_uint32()
+        _check(self.L.ohpt_scene_tree_depth(self.h, C.byref(d)), "ohpt_scene_tree_depth")
+        return d.value
+
+    def export_bvh(self, n):
+        ni = max(n - 1, 1)
+        nodes = np.zeros((ni, 16), np.uint32)
+        tris = np.zeros((n, 12), np.float32)
+        prim = np.zeros(n, np.uint32)
+        _check(self.L.ohpt_scene_export_bvh(self.h, nodes.ctypes.data, tris.ctypes.data,
+                                            prim.ctypes.data_as(_u32p)), "ohpt_scene_export_bvh")
+        return nodes, tris, prim
+
+    def set_materials(self, mats):
+        m = np.ascontiguousarray(mats, S.MATERIAL_DTYPE)
+        self._keep.append(m)
+        _check(self.L.ohpt_upload_materials(self.h, m.ctypes.data, m.shape[0]), "ohpt_upload_materials")
+
+    def upload_texture(self, tex) -> int:
+        t = np.ascontiguousarray(tex, np.float32)
+        if t.ndim == 2:
+            t = t[..., None]
+        tid = C.c_int32()
+        _check(self.L.ohpt_upload_texture(self.h, C.byref(tid), _fp(t), t.shape[1], t.shape[0], t.shape[2]),
+               "ohpt_upload_texture")
+        return tid.value
+
+    def update_texture(self, tid, tex):
+        if hasattr(tex, "data_ptr"):
+            _check(self.L.ohpt_update_texture(self.h, tid, C.cast(tex.data_ptr(), _f32p), 1),
+                   "ohpt_update_texture")
+        else:
+            t = np.ascontiguousarray(tex, np.float32)
+            _check(self.L.ohpt_update_texture(self.h, tid, _fp(t), 0), "ohpt_update_texture")
+
+    def set_lights(self, lights):
+        l = np.ascontiguousarray(lights, np.float32).reshape(-1, 20)
+        self._keep.append(l)
+        _check(self.L.ohpt_upload_lights(self.h, l.ctypes.data if l.shape[0] else None, l.shape[0]),
+               "ohpt_upload_lights")
+
+    def set_env(self, env):
+        if env is None:
+            _check(self.L.ohpt_upload_env(self.h, None, 0, 0), "ohpt_upload_env")
+            return
+        e = np.ascontiguousarray(env, np.float32)
+        _check(self.L.ohpt_upload_env(self.h, _fp(e), e.shape[1], e.shape[0]), "ohpt_upload_env")
+
+    def set_env_intensity(self, s):
+        _check(self.L.ohpt_set_env_intensity(self.h, float(s)), "ohpt_set_env_intensity")
+
+    def export_env_cdf(self, W, H):
+        m = np.zeros(H, np.float32)
+        c = np.zeros((H, W), np.float32)
+        i = np.zeros(1, np.float32)
+        _check(self.L.ohpt_export_env_cdf(self.h, _fp(m), _fp(c), _fp(i)), "ohpt_export_env_cdf")
+        return m, c, float(i[0])
+
+    def set_camera(self, cam: "S.Camera"):
+        c = Camera()
+        c.position[:] = [float(x) for x in cam.position]
+        c.forward[:] = [float(x) for x in cam.forward]
+        c.right[:] = [float(x) for x in cam.right]
+        c.up[:] = [float(x) for x in cam.up]
+        c.tan_half_fov_y = cam.tan_half_fov_y
+        _check(self.L.ohpt_set_camera(self.h, C.byref(c)), "ohpt_set_camera")
+
+    def set_settings(self, s: "S.Settings"):
+        st = Settings()
+        st.max_bounces, st.rr_start, st.sampler, st.flags = s.max_bounces, s.rr_start, s.sampler, s.flags
+        st.firefly_clamp, st.ray_eps = s.firefly_clamp, s.ray_eps
+        _check(self.L.ohpt_set_settings(self.h, C.byref(st)), "ohpt_set_settings")
+
+    def set_render_seed(self, seed: int):
+        _check(self.L.ohpt_set_render_seed(self.h, seed), "ohpt_set_render_seed")
+
+    def reset(self):
+        _check(self.L.ohpt_reset_accumulation(self.h), "ohpt_reset_accumulation")
+
+    # -- rendering --------------------------------------------------------------
+    def render(self, spp: int, flags: int = 0):
+        _check(self.L.ohpt_render(self.h, spp, flags), "ohpt_render")
+
+    def synchronize(self):
+        _check(self.L.ohpt_synchronize(self.h), "ohpt_synchronize")
+
+    def stats(self) -> Stats:
+        st = Stats()
+        _check(self.L.ohpt_get_stats(self.h, C.byref(st)), "ohpt_get_stats")
+        return st
+
+    def read_accum(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        _check(self.L.ohpt_read_accum(self.h, _fp(out), 0), "ohpt_read_accum")
+        return out
+
+    def read_accum_device(self, t):
+        _check(self.L.ohpt_read_accum(self.h, C.cast(t.data_ptr(), _f32p), 1), "ohpt_read_accum")
+
+    def read_moments(self):
+        s = np.zeros((self.height, self.width, 4), np.float32)
+        q = np.zeros((self.height, self.width, 4), np.float32)
+        _check(self.L.ohpt_read_moments(self.h, _fp(s), _fp(q), 0), "ohpt_read_moments")
+        return s, q
+
+    def read_pixels(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        _check(self.L.ohpt_read_pixels(self.h, out.ctypes.data_as(_u8p), 0), "ohpt_read_pixels")
+        return out
+
+    def read_aovs(self):
+        a = np.zeros((self.height, self.width, 4), np.float32)
+        n = np.zeros((self.height, self.width, 4), np.float32)
+        d = np.zeros((self.height, self.width), np.float32)
+        _check(self.L.ohpt_read_aovs(self.h, _fp(a), _fp(n), _fp(d), 0), "ohpt_read_aovs")
+        return a, n, d
+
+    def trace_rays(self, rays: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((r.shape[0], 4), np.float32)
+        _check(self.L.ohpt_trace_rays(self.h, _fp(r), _fp(hits), r.shape[0], 0), "ohpt_trace_rays")
+        return hits
+
+    def occluded_rays(self, rays: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(r.shape[0], np.uint32)
+        _check(self.L.ohpt_occluded_rays(self.h, _fp(r), out.ctypes.data_as(_u32p), r.shape[0], 0),
+               "ohpt_occluded_rays")
+        return out
+
+
+def context_for(sc: "S.Scene", device: int = 0) -> Context:
+    ctx = Context(sc.width, sc.height, device)
+    ctx.load_scene(sc)
+    return ctx

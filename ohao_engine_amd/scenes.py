@@ -1,0 +1,448 @@
+"""Synthetic scene generators for the workloads the path tracer is measured on.
+
+Scenes are plain numpy arrays of world-space triangles (flattened instances),
+materials in the `ohpt_material` layout, reference `GPULight` records, an
+RGBA32F equirect environment (row 0 = +Y) and a camera in the reference's
+yaw/pitch convention (ohao/render/camera/camera.cpp:25-47).
+
+Workloads:
+  cornell_box()   0.55 m Cornell room, 2-triangle (17,12,4) area light, 256x256,
+                  64 spp, max depth 8, RR from bounce 3, PCG32, seed 1.
+  procedural()    64 displaced geodesic spheres (32,000 triangles each) over a
+                  1024x512-cell heightfield (1,048,576 triangles): 3,096,576
+                  triangles; GGX metallic-roughness materials; 2048x1024 sky
+                  gradient + 0.5 degree sun disc at 2e4; 1920x1080, 1 spp,
+                  depth 4, seed 7.
+  lantern()       displaced-sphere lantern (200,000 triangles) on a plane with a
+                  1024x1024 RGB checker albedo texture (16x16 tiles), 8 cameras on
+                  a 30 degree ring at 512x512, 16 spp, depth 4.
+  furnace_sphere(), enclosure()   closed-form test scenes.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+MATERIAL_DTYPE = np.dtype([
+    ("base_color", "<f4", 3), ("roughness", "<f4"), ("metallic", "<f4"),
+    ("specular_f0", "<f4"), ("emissive", "<f4", 3), ("albedo_tex", "<i4"),
+    ("roughness_tex", "<i4"), ("metallic_tex", "<i4"), ("reserved", "<i4", 4)])
+assert MATERIAL_DTYPE.itemsize == 64
+
+LIGHT_SPHERE, LIGHT_DIRECTIONAL, LIGHT_SPOT, LIGHT_AREA_RECT = 0, 1, 2, 3
+SAMPLER_PCG32, SAMPLER_SOBOL_OWEN = 0, 1
+FLAG_AOVS, FLAG_FIREFLY_CLAMP, FLAG_NO_ENV_NEE, FLAG_NO_LIGHT_NEE = 1, 4, 256, 512
+
+
+def material(base=(0.8, 0.8, 0.8), roughness=0.5, metallic=0.0, specular_f0=0.04,
+             emissive=(0.0, 0.0, 0.0), albedo_tex=-1, roughness_tex=-1, metallic_tex=-1):
+    m = np.zeros((), MATERIAL_DTYPE)
+    m["base_color"] = base
+    m["roughness"] = roughness
+    m["metallic"] = metallic
+    m["specular_f0"] = specular_f0
+    m["emissive"] = emissive
+    m["albedo_tex"] = albedo_tex
+    m["roughness_tex"] = roughness_tex
+    m["metallic_tex"] = metallic_tex
+    m["reserved"] = 0
+    return m
+
+
+def lambert(albedo, emissive=(0.0, 0.0, 0.0)):
+    return material(albedo, roughness=1.0, metallic=0.0, specular_f0=0.0, emissive=emissive)
+
+
+@dataclass
+class Camera:
+    position: np.ndarray
+    forward: np.ndarray
+    right: np.ndarray
+    up: np.ndarray
+    tan_half_fov_y: float
+
+    @staticmethod
+    def from_yaw_pitch(position, yaw_deg, pitch_deg, fov_y_deg):
+        """Camera::updateVectors (camera.cpp:25-40) + perspective fov."""
+        yaw, pitch = math.radians(yaw_deg), math.radians(max(-89.0, min(89.0, pitch_deg)))
+        f = np.array([math.cos(yaw) * math.cos(pitch), math.sin(pitch),
+                      math.sin(yaw) * math.cos(pitch)], np.float64)
+        f /= np.linalg.norm(f)
+        r = np.cross(f, np.array([0.0, 1.0, 0.0]))
+        r /= np.linalg.norm(r)
+        u = np.cross(r, f)
+        u /= np.linalg.norm(u)
+        return Camera(np.asarray(position, np.float32), f.astype(np.float32), r.astype(np.float32),
+                      u.astype(np.float32), float(math.tan(math.radians(fov_y_deg) * 0.5)))
+
+    @staticmethod
+    def look_at(position, target, fov_y_deg):
+        d = np.asarray(target, np.float64) - np.asarray(position, np.float64)
+        d /= np.linalg.norm(d)
+        yaw = math.degrees(math.atan2(d[2], d[0]))
+        pitch = math.degrees(math.asin(max(-1.0, min(1.0, d[1]))))
+        return Camera.from_yaw_pitch(position, yaw, pitch, fov_y_deg)
+
+
+@dataclass
+class Settings:
+    max_bounces: int = 4
+    rr_start: int = 3
+    sampler: int = SAMPLER_PCG32
+    flags: int = 0
+    firefly_clamp: float = 0.0
+    ray_eps: float = 0.0
+    seed: int = 1
+    spp: int = 1
+
+
+@dataclass
+class Scene:
+    pos: np.ndarray                      # (n, 9) float32 world-space triangles
+    mat: np.ndarray                      # (n,) uint32
+    materials: np.ndarray                # (m,) MATERIAL_DTYPE
+    nrm: Optional[np.ndarray] = None     # (n, 9) float32 per-corner normals
+    uv: Optional[np.ndarray] = None      # (n, 6) float32 per-corner uvs
+    textures: List[np.ndarray] = field(default_factory=list)   # (h, w, c) float32
+    lights: np.ndarray = field(default_factory=lambda: np.zeros((0, 20), np.float32))
+    env: Optional[np.ndarray] = None     # (h, w, 4) float32, row 0 = +Y
+    env_intensity: float = 1.0
+    camera: Optional[Camera] = None
+    width: int = 256
+    height: int = 256
+    settings: Settings = field(default_factory=Settings)
+    name: str = ""
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.pos.shape[0])
+
+
+# ---------------------------------------------------------------- geometry helpers
+def _quad(a, b, c, d):
+    """Two triangles (a,b,c), (a,c,d); normal = normalize((b-a) x (c-a))."""
+    return [np.concatenate([a, b, c]), np.concatenate([a, c, d])]
+
+
+def _box_tris(center, size, rot_y_deg, top=True, bottom=False):
+    """Axis box rotated about +y; faces wound outward. size=(sx, sy, sz), base at y=center[1]-sy/2."""
+    cx, cy, cz = center
+    sx, sy, sz = [s * 0.5 for s in size]
+    th = math.radians(rot_y_deg)
+    c, s = math.cos(th), math.sin(th)
+
+    def P(x, y, z):
+        return np.array([cx + c * x + s * z, cy + y, cz - s * x + c * z], np.float64)
+
+    v = {k: P(*k) for k in [(x, y, z) for x in (-sx, sx) for y in (-sy, sy) for z in (-sz, sz)]}
+    tris = []
+    tris += _quad(v[(-sx, -sy, sz)], v[(sx, -sy, sz)], v[(sx, sy, sz)], v[(-sx, sy, sz)])      # +z
+    tris += _quad(v[(sx, -sy, -sz)], v[(-sx, -sy, -sz)], v[(-sx, sy, -sz)], v[(sx, sy, -sz)])  # -z
+    tris += _quad(v[(sx, -sy, sz)], v[(sx, -sy, -sz)], v[(sx, sy, -sz)], v[(sx, sy, sz)])      # +x
+    tris += _quad(v[(-sx, -sy, -sz)], v[(-sx, -sy, sz)], v[(-sx, sy, sz)], v[(-sx, sy, -sz)])  # -x
+    if top:
+        tris += _quad(v[(-sx, sy, sz)], v[(sx, sy, sz)], v[(sx, sy, -sz)], v[(-sx, sy, -sz)])  # +y
+    if bottom:
+        tris += _quad(v[(-sx, -sy, -sz)], v[(sx, -sy, -sz)], v[(sx, -sy, sz)], v[(-sx, -sy, sz)])
+    return tris
+
+
+def _tri_normals(pos):
+    v0, v1, v2 = pos[:, 0:3], pos[:, 3:6], pos[:, 6:9]
+    n = np.cross(v1 - v0, v2 - v0)
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+def geodesic_sphere(freq: int):
+    """Icosahedron with every face split into freq^2 triangles, projected to the
+    unit sphere: 20*freq^2 triangles (freq=40 gives 32,000). Returns (V, F)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    ico_v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+                      [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    ico_f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+                      [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8],
+                      [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    n = freq
+    ij = [(i, j) for i in range(n + 1) for j in range(n + 1 - i)]
+    idx = {p: k for k, p in enumerate(ij)}
+    loc = np.array(ij, np.float64)
+    tri_local = []
+    for i in range(n):
+        for j in range(n - i):
+            tri_local.append((idx[(i, j)], idx[(i + 1, j)], idx[(i, j + 1)]))
+            if j < n - i - 1:
+                tri_local.append((idx[(i + 1, j)], idx[(i + 1, j + 1)], idx[(i, j + 1)]))
+    tri_local = np.array(tri_local, np.int64)
+    verts, faces = [], []
+    base = 0
+    for a, b, c in ico_f:
+        A, B, C = ico_v[a], ico_v[b], ico_v[c]
+        P = A[None] + (loc[:, 0:1] / n) * (B - A)[None] + (loc[:, 1:2] / n) * (C - A)[None]
+        P /= np.linalg.norm(P, axis=1, keepdims=True)
+        verts.append(P)
+        faces.append(tri_local + base)
+        base += P.shape[0]
+    V = np.concatenate(verts)
+    F = np.concatenate(faces)
+    key = np.round(V * 1e6).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    V = V[first]
+    F = inv.reshape(-1)[F]
+    return V, F
+
+
+def _hash3(ix, iy, iz, seed):
+    h = (ix.astype(np.uint64) * np.uint64(0x8DA6B343) ^ iy.astype(np.uint64) * np.uint64(0xD8163841)
+         ^ iz.astype(np.uint64) * np.uint64(0xCB1AB31F) ^ np.uint64(seed) * np.uint64(0x9E3779B9))
+    h &= np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(16)
+    h = (h * np.uint64(0x7FEB352D)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(15)
+    h = (h * np.uint64(0x846CA68B)) & np.uint64(0xFFFFFFFF)
+    h ^= h >> np.uint64(16)
+    return h.astype(np.float64) / 4294967295.0 * 2.0 - 1.0
+
+
+def value_noise3(p, seed):
+    """Lattice value noise in [-1, 1] with smoothstep trilinear interpolation."""
+    f = np.floor(p)
+    t = p - f
+    w = t * t * (3.0 - 2.0 * t)
+    i = f.astype(np.int64)
+    out = 0.0
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                v = _hash3(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz, seed)
+                wx = w[:, 0] if dx else 1.0 - w[:, 0]
+                wy = w[:, 1] if dy else 1.0 - w[:, 1]
+                wz = w[:, 2] if dz else 1.0 - w[:, 2]
+                out = out + v * wx * wy * wz
+    return out
+
+
+def fbm3(p, seed, octaves=4, base_freq=2.0):
+    total, amp, norm = 0.0, 1.0, 0.0
+    for o in range(octaves):
+        total = total + amp * value_noise3(p * (base_freq * 2.0 ** o), seed + 101 * o)
+        norm += amp
+        amp *= 0.5
+    return total / norm
+
+
+def vertex_normals(V, F):
+    fn = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    N = np.zeros_like(V)
+    for k in range(3):
+        np.add.at(N, F[:, k], fn)
+    return N / np.maximum(np.linalg.norm(N, axis=1, keepdims=True), 1e-20)
+
+
+def displaced_sphere(freq, amplitude, seed):
+    V, F = geodesic_sphere(freq)
+    r = 1.0 + amplitude * fbm3(V, seed)
+    Vd = V * r[:, None]
+    N = vertex_normals(Vd, F)
+    th = np.arccos(np.clip(V[:, 1], -1, 1))
+    ph = np.arctan2(V[:, 2], V[:, 0])
+    UV = np.stack([ph / (2 * np.pi) + 0.5, th / np.pi], axis=1)
+    return Vd, N, UV, F
+
+
+def flatten(V, N, UV, F, scale=1.0, rot_y=0.0, offset=(0, 0, 0)):
+    c, s = math.cos(rot_y), math.sin(rot_y)
+    R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]], np.float64)
+    Vw = (V * scale) @ R.T + np.asarray(offset, np.float64)
+    Nw = N @ R.T
+    pos = Vw[F].reshape(-1, 9).astype(np.float32)
+    nrm = Nw[F].reshape(-1, 9).astype(np.float32)
+    uv = UV[F].reshape(-1, 6).astype(np.float32) if UV is not None else None
+    return pos, nrm, uv
+
+
+def heightfield(nx, nz, x0, x1, z0, z1, amp, seed):
+    xs = np.linspace(x0, x1, nx + 1)
+    zs = np.linspace(z0, z1, nz + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="xy")
+    P = np.stack([X.ravel() * 0.15, Z.ravel() * 0.15, np.zeros(X.size)], axis=1)
+    Y = amp * fbm3(P, seed, octaves=4, base_freq=1.0)
+    V = np.stack([X.ravel(), Y, Z.ravel()], axis=1)
+    idx = np.arange((nx + 1) * (nz + 1)).reshape(nz + 1, nx + 1)
+    a, b = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel()
+    c, d = idx[1:, 1:].ravel(), idx[1:, :-1].ravel()
+    # wound so the normal points +y
+    F = np.concatenate([np.stack([a, d, c], 1), np.stack([a, c, b], 1)])
+    UV = np.stack([(V[:, 0] - x0) / (x1 - x0), (V[:, 2] - z0) / (z1 - z0)], axis=1)
+    return V, vertex_normals(V, F), UV, F, Y.reshape(nz + 1, nx + 1)
+
+
+# ---------------------------------------------------------------- environment maps
+def sky_env(width=2048, height=1024, sun_dir=None, sun_deg=0.5, sun_radiance=2.0e4,
+            low=0.2, high=1.0):
+    """Sky gradient low (nadir) -> high (zenith) plus a sun disc of angular
+    diameter sun_deg. Texel (x, y) centre direction follows equirectPixelToDir
+    (env_sampling.glsl:40-49): row 0 = +Y."""
+    y = (np.arange(height) + 0.5) / height * np.pi
+    x = ((np.arange(width) + 0.5) / width - 0.5) * 2 * np.pi
+    TH, PH = np.meshgrid(y, x, indexing="ij")
+    D = np.stack([np.sin(TH) * np.cos(PH), np.cos(TH), np.sin(TH) * np.sin(PH)], axis=-1)
+    g = low + (high - low) * 0.5 * (D[..., 1] + 1.0)
+    env = np.zeros((height, width, 4), np.float32)
+    env[..., 0] = g
+    env[..., 1] = g
+    env[..., 2] = g
+    env[..., 3] = 1.0
+    if sun_dir is not None:
+        s = np.asarray(sun_dir, np.float64)
+        s /= np.linalg.norm(s)
+        cosr = math.cos(math.radians(sun_deg * 0.5))
+        mask = (D @ s) >= cosr
+        env[mask, 0:3] = sun_radiance
+    return env
+
+
+def constant_env(value=1.0, width=64, height=32):
+    env = np.zeros((height, width, 4), np.float32)
+    env[..., 0:3] = value
+    env[..., 3] = 1.0
+    return env
+
+
+# ---------------------------------------------------------------- workloads
+def cornell_box(width=256, height=256, spp=64, seed=1) -> Scene:
+    """0.555 m Cornell room. The room is closed (its sixth wall sits just behind
+    the camera) so no light leaves it; boxes rotated -17 / +15 degrees about y."""
+    W = 0.555
+    zf = -0.81
+    tris, mats = [], []
+    white, red, green, light = 0, 1, 2, 3
+    P = lambda x, y, z: np.array([x, y, z], np.float64)
+    # room walls wound with normals pointing into the room
+    tris += _quad(P(0, 0, zf), P(0, 0, W), P(W, 0, W), P(W, 0, zf)); mats += [white] * 2    # floor +y
+    tris += _quad(P(0, W, zf), P(W, W, zf), P(W, W, W), P(0, W, W)); mats += [white] * 2    # ceiling -y
+    tris += _quad(P(0, 0, W), P(0, W, W), P(W, W, W), P(W, 0, W)); mats += [white] * 2      # back -z
+    tris += _quad(P(W, 0, zf), P(W, 0, W), P(W, W, W), P(W, W, zf)); mats += [red] * 2      # left (x=W) -x
+    tris += _quad(P(0, 0, zf), P(0, W, zf), P(0, W, W), P(0, 0, W)); mats += [green] * 2    # right (x=0) +x
+    tris += _quad(P(0, 0, zf), P(W, 0, zf), P(W, W, zf), P(0, W, zf)); mats += [white] * 2  # front +z
+    sb = _box_tris((0.1855, 0.0825, 0.169), (0.165, 0.165, 0.165), -17.0)
+    tb = _box_tris((0.3685, 0.165, 0.3513), (0.165, 0.33, 0.165), 15.0)
+    tris += sb; mats += [white] * len(sb)
+    tris += tb; mats += [white] * len(tb)
+    ly = W - 1e-3
+    tris += _quad(P(0.213, ly, 0.227), P(0.343, ly, 0.227), P(0.343, ly, 0.332), P(0.213, ly, 0.332))
+    mats += [light] * 2
+    pos = np.array(tris, np.float32)
+    materials = np.array([lambert((0.725, 0.71, 0.68)), lambert((0.63, 0.065, 0.05)),
+                          lambert((0.14, 0.45, 0.091)),
+                          lambert((0.78, 0.78, 0.78), emissive=(17.0, 12.0, 4.0))], MATERIAL_DTYPE)
+    cam = Camera.from_yaw_pitch((0.278, 0.273, -0.8), 90.0, 0.0, 39.3)
+    return Scene(pos=pos, mat=np.array(mats, np.uint32), materials=materials, camera=cam,
+                 width=width, height=height, name="cornell_box",
+                 settings=Settings(max_bounces=8, rr_start=3, sampler=SAMPLER_PCG32, seed=seed, spp=spp))
+
+
+def procedural(width=1920, height=1080, seed=7, sphere_freq=40, hf_cells=(1024, 512),
+               n_instances=64, env_size=(2048, 1024)) -> Scene:
+    rng = np.random.default_rng(seed)
+    V, N, UV, F = displaced_sphere(sphere_freq, 0.1, seed)
+    x0, x1, z0, z1 = -24.0, 24.0, -12.0, 12.0
+    Vh, Nh, UVh, Fh, Y = heightfield(hf_cells[0], hf_cells[1], x0, x1, z0, z1, 0.6, seed + 1)
+    pos_l, nrm_l, uv_l, mat_l = [], [], [], []
+    p, n, u = flatten(Vh, Nh, UVh, Fh)
+    pos_l.append(p); nrm_l.append(n); uv_l.append(u); mat_l.append(np.zeros(p.shape[0], np.uint32))
+    mats = [material((0.5, 0.5, 0.5), roughness=0.9, metallic=0.0)]
+    side = int(round(math.sqrt(n_instances)))
+    for k in range(n_instances):
+        i, j = k % side, k // side
+        cx = -17.5 + 5.0 * i + rng.uniform(-0.6, 0.6)
+        cz = -9.1 + 2.6 * j + rng.uniform(-0.2, 0.2)
+        gx = int(np.clip((cx - x0) / (x1 - x0) * hf_cells[0], 0, hf_cells[0]))
+        gz = int(np.clip((cz - z0) / (z1 - z0) * hf_cells[1], 0, hf_cells[1]))
+        cy = float(Y[gz, gx]) + 1.05
+        p, n, u = flatten(V, N, UV, F, 1.0, rng.uniform(0, 2 * np.pi), (cx, cy, cz))
+        pos_l.append(p); nrm_l.append(n); uv_l.append(u)
+        mat_l.append(np.full(p.shape[0], len(mats), np.uint32))
+        mats.append(material(tuple(rng.uniform(0.1, 0.9, 3)), roughness=float(rng.uniform(0.05, 1.0)),
+                             metallic=float(rng.random() < 0.3)))
+    sun = (math.cos(math.radians(35)) * math.cos(math.radians(60)), math.sin(math.radians(35)),
+           math.cos(math.radians(35)) * math.sin(math.radians(60)))
+    env = sky_env(env_size[0], env_size[1], sun_dir=sun)
+    cam = Camera.look_at((0.0, 7.0, 21.0), (0.0, 0.0, -1.0), 50.0)
+    return Scene(pos=np.concatenate(pos_l), nrm=np.concatenate(nrm_l), uv=np.concatenate(uv_l),
+                 mat=np.concatenate(mat_l), materials=np.array(mats, MATERIAL_DTYPE), env=env,
+                 camera=cam, width=width, height=height, name="procedural",
+                 settings=Settings(max_bounces=4, rr_start=3, sampler=SAMPLER_PCG32, seed=seed, spp=1))
+
+
+def checker_texture(size=1024, tiles=16, seed=3, low=0.1, high=0.9):
+    rng = np.random.default_rng(seed)
+    cols = rng.uniform(low, high, (tiles, tiles, 3)).astype(np.float32)
+    t = size // tiles
+    return np.repeat(np.repeat(cols, t, axis=0), t, axis=1)
+
+
+def ring_cameras(n=8, elevation_deg=30.0, radius=4.0, target=(0.0, 1.0, 0.0), fov=40.0):
+    cams = []
+    for k in range(n):
+        a = 2 * math.pi * k / n
+        el = math.radians(elevation_deg)
+        pos = (target[0] + radius * math.cos(el) * math.cos(a), target[1] + radius * math.sin(el),
+               target[2] + radius * math.cos(el) * math.sin(a))
+        cams.append(Camera.look_at(pos, target, fov))
+    return cams
+
+
+def lantern(width=512, height=512, spp=16, seed=11, freq=100, tex_size=1024, texture=None) -> Scene:
+    """Studio scene: displaced-sphere lantern (20*freq^2 triangles) on a plane,
+    albedo texture on the lantern, sky env + a rectangular area light."""
+    V, N, UV, F = displaced_sphere(freq, 0.08, seed)
+    p, n, u = flatten(V, N, UV, F, 1.0, 0.0, (0.0, 1.0, 0.0))
+    Pq = lambda x, y, z: np.array([x, y, z], np.float64)
+    plane = np.array(_quad(Pq(-6, -0.1, -6), Pq(-6, -0.1, 6), Pq(6, -0.1, 6), Pq(6, -0.1, -6)), np.float32)
+    pos = np.concatenate([p, plane])
+    nrm = np.concatenate([n, np.tile(np.array([0, 1, 0] * 3, np.float32), (2, 1))])
+    uv = np.concatenate([u, np.zeros((2, 6), np.float32)])
+    mat = np.concatenate([np.zeros(p.shape[0], np.uint32), np.ones(2, np.uint32)])
+    tex = checker_texture(tex_size, 16, seed) if texture is None else texture
+    mats = np.array([material((1, 1, 1), roughness=0.6, metallic=0.0, albedo_tex=0),
+                     material((0.6, 0.6, 0.6), roughness=0.8, metallic=0.0)], MATERIAL_DTYPE)
+    light = np.zeros((1, 20), np.float32)
+    light[0, 0:4] = (-1.0, 4.0, -1.0, LIGHT_AREA_RECT)
+    light[0, 4:8] = (1.0, 0.97, 0.9, 6.0)
+    light[0, 8:12] = (0, -1, 0, 0)
+    light[0, 12:16] = (2.0, 0.0, 0.0, 0.0)
+    light[0, 16:20] = (0.0, 0.0, 2.0, 4.0)
+    env = sky_env(512, 256, sun_dir=None, low=0.05, high=0.6)
+    cam = ring_cameras()[0]
+    return Scene(pos=pos, nrm=nrm, uv=uv, mat=mat, materials=mats, textures=[tex], lights=light,
+                 env=env, camera=cam, width=width, height=height, name="lantern",
+                 settings=Settings(max_bounces=4, rr_start=1000, sampler=SAMPLER_PCG32, seed=seed, spp=spp))
+
+
+def furnace_sphere(albedo=(0.5, 0.5, 0.5), roughness=1.0, metallic=0.0, specular_f0=0.0,
+                   freq=16, width=64, height=64, env_value=1.0) -> Scene:
+    """Convex sphere with flat normals under a constant environment. For a
+    Lambert surface every pixel on the sphere converges to albedo*env_value."""
+    V, F = geodesic_sphere(freq)
+    pos = V[F].reshape(-1, 9).astype(np.float32)
+    mats = np.array([material(albedo, roughness, metallic, specular_f0)], MATERIAL_DTYPE)
+    cam = Camera.look_at((0.0, 0.0, 3.2), (0.0, 0.0, 0.0), 40.0)
+    return Scene(pos=pos, mat=np.zeros(pos.shape[0], np.uint32), materials=mats,
+                 env=constant_env(env_value), camera=cam, width=width, height=height,
+                 name="furnace_sphere", settings=Settings(max_bounces=6, rr_start=1000, seed=5))
+
+
+def enclosure(Le=(1.0, 0.5, 0.25), rho=0.5, max_bounces=3, width=32, height=32) -> Scene:
+    """Closed box, every wall Lambert (rho) and emitting Le towards the inside.
+    Radiance everywhere: Le * sum_{k=0..D} rho^k with D = max_bounces."""
+    tris = _box_tris((0.0, 0.0, 0.0), (2.0, 2.0, 2.0), 0.0, top=True, bottom=True)
+    pos = np.array(tris, np.float32)
+    pos = pos.reshape(-1, 3, 3)[:, ::-1, :].reshape(-1, 9).copy()   # flip winding: normals inward
+    mats = np.array([lambert((rho, rho, rho), emissive=Le)], MATERIAL_DTYPE)
+    cam = Camera.look_at((0.1, 0.05, 0.2), (0.3, -0.2, -1.0), 60.0)
+    return Scene(pos=pos, mat=np.zeros(pos.shape[0], np.uint32), materials=mats, camera=cam,
+                 width=width, height=height, name="enclosure",
+                 settings=Settings(max_bounces=max_bounces, rr_start=1000, seed=9))

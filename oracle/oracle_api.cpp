@@ -1,0 +1,176 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY. extern "C" surface of the CPU
+// restatement for tests/ (ctypes), smoke() and bench.py's cpu_baseline leg.
+#include "oracle_render.hpp"
+
+#include <cstring>
+
+using namespace oracle;
+
+extern "C" {
+
+// ---- samplers ---------------------------------------------------------------
+void oracle_pcg32_seq(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out) {
+    Pcg32 r;
+    r.seed(initstate, initseq);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next();
+}
+uint32_t oracle_sobol_int(uint32_t index, uint32_t dim) { return sobol_int(index, dim); }
+uint32_t oracle_owen(uint32_t v, uint32_t seed) { return owen_scramble(v, seed); }
+uint32_t oracle_hash_pixel(uint32_t x, uint32_t y) { return hash_pixel(x, y); }
+float oracle_sobol_sample(uint32_t px, uint32_t py, uint32_t index, uint32_t dim) {
+    return sobol_owen_sample(px, py, index, dim);
+}
+float oracle_sobol_unscrambled(uint32_t index, uint32_t dim) { return u32_to_unit(sobol_int(index, dim)); }
+void oracle_draw_camera(uint32_t sampler, uint32_t px, uint32_t py, uint32_t width, uint32_t sidx,
+                        float* out2) {
+    draw_camera(sampler, px, py, width, sidx, out2);
+}
+void oracle_draw_vertex(uint32_t sampler, uint32_t px, uint32_t py, uint32_t width, uint32_t sidx,
+                        uint32_t vertex, float* out9) {
+    VertexSamples vs;
+    draw_vertex(sampler, px, py, width, sidx, vertex, vs);
+    std::memcpy(out9, vs.u, sizeof(vs.u));
+}
+
+// ---- env CDF ---------------------------------------------------------------
+void oracle_env_cdf(const float* rgba, int W, int H, float* marg, float* cond, float* integral) {
+    std::vector<float> m, c;
+    float I = 0;
+    env_cdf_build(rgba, W, H, m, c, I);
+    std::memcpy(marg, m.data(), m.size() * 4);
+    std::memcpy(cond, c.data(), c.size() * 4);
+    *integral = I;
+}
+
+uint32_t oracle_morton30(float cx, float cy, float cz, const float* cmin, const float* cmax) {
+    return morton30(cx, cy, cz, cmin, cmax);
+}
+
+// ---- scene -------------------------------------------------------------------
+void* oracle_scene_create(void) { return new Scene(); }
+void oracle_scene_destroy(void* h) { delete static_cast<Scene*>(h); }
+
+void oracle_scene_set_geometry(void* h, uint32_t n, const float* pos, const float* nrm,
+                               const float* uv, const uint32_t* mat) {
+    Scene& s = *static_cast<Scene*>(h);
+    s.n = n;
+    s.pos.assign(pos, pos + (size_t)n * 9);
+    if (nrm) s.nrm.assign(nrm, nrm + (size_t)n * 9); else s.nrm.clear();
+    if (uv) s.uv.assign(uv, uv + (size_t)n * 6); else s.uv.clear();
+    if (mat) s.mat.assign(mat, mat + n); else s.mat.assign(n, 0u);
+    s.bvh_ready = false;
+}
+void oracle_scene_set_materials(void* h, const ohpt_material* m, uint32_t count) {
+    Scene& s = *static_cast<Scene*>(h);
+    s.materials.assign(m, m + count);
+}
+int oracle_scene_add_texture(void* h, uint32_t w, uint32_t hh, uint32_t c, const float* data) {
+    Scene& s = *static_cast<Scene*>(h);
+    Texture t;
+    t.w = w; t.h = hh; t.c = c;
+    t.data.assign(data, data + (size_t)w * hh * c);
+    s.textures.push_back(std::move(t));
+    s.textures_eval.clear();
+    return (int)s.textures.size() - 1;
+}
+// Evaluation-side copy of texture `id` (FD of the detached estimator); data = NULL resets.
+void oracle_scene_set_texture_eval(void* h, int id, const float* data) {
+    Scene& s = *static_cast<Scene*>(h);
+    if (!data) { s.textures_eval.clear(); return; }
+    if (s.textures_eval.empty()) s.textures_eval = s.textures;
+    Texture& t = s.textures_eval[(size_t)id];
+    std::memcpy(t.data.data(), data, t.data.size() * 4);
+}
+void oracle_scene_update_texture(void* h, int id, const float* data) {
+    Scene& s = *static_cast<Scene*>(h);
+    Texture& t = s.textures[(size_t)id];
+    std::memcpy(t.data.data(), data, t.data.size() * 4);
+}
+void oracle_scene_set_lights(void* h, const ohpt_light* l, uint32_t count) {
+    Scene& s = *static_cast<Scene*>(h);
+    s.lights.assign(l, l + count);
+}
+void oracle_scene_set_env(void* h, const float* rgba, uint32_t w, uint32_t hh, float intensity) {
+    Scene& s = *static_cast<Scene*>(h);
+    s.env_w = rgba ? w : 0;
+    s.env_h = rgba ? hh : 0;
+    if (rgba) s.env.assign(rgba, rgba + (size_t)w * hh * 4); else s.env.clear();
+    s.env_intensity = intensity;
+    if (s.env_w) env_cdf_build(s.env.data(), (int)w, (int)hh, s.env_marg, s.env_cond, s.env_integral);
+}
+void oracle_scene_set_camera(void* h, const ohpt_camera* cam, uint32_t W, uint32_t H) {
+    Scene& s = *static_cast<Scene*>(h);
+    s.cam = *cam;
+    s.width = W;
+    s.height = H;
+}
+void oracle_scene_set_settings(void* h, const ohpt_settings* st) {
+    static_cast<Scene*>(h)->st = *st;
+}
+void oracle_scene_commit(void* h) {
+    Scene& s = *static_cast<Scene*>(h);
+    lbvh_build(s.pos.data(), s.n, s.bvh);
+    s.bvh_ready = true;
+    s.build_emitters();
+}
+void oracle_scene_rebuild_emitters(void* h) { static_cast<Scene*>(h)->build_emitters(); }
+
+uint32_t oracle_scene_bvh(void* h, void* nodes_out, float* tris_out, uint32_t* prim_out,
+                          uint64_t* keys_out) {
+    const Scene& s = *static_cast<Scene*>(h);
+    const Bvh& b = s.bvh;
+    if (nodes_out) std::memcpy(nodes_out, b.nodes.data(), b.nodes.size() * sizeof(BvhNode));
+    if (tris_out) std::memcpy(tris_out, b.tri.data(), b.tri.size() * 4);
+    if (prim_out) std::memcpy(prim_out, b.prim.data(), b.prim.size() * 4);
+    if (keys_out) std::memcpy(keys_out, b.keys.data(), b.keys.size() * 8);
+    return (uint32_t)b.nodes.size();
+}
+
+// rays: n x 8 (ox,oy,oz,tmin,dx,dy,dz,tmax); hits: n x 4 (t, prim bits, u, v)
+void oracle_trace_batch(void* h, const float* rays, uint32_t n, float* hits, uint32_t* visits,
+                        uint32_t* tests) {
+    const Scene& s = *static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + (size_t)i * 8;
+        uint32_t nv = 0, nt = 0;
+        const HitF32 hh = trace_f32(s.bvh, r, r + 4, r[3], r[7], &nv, &nt);
+        float* o = hits + (size_t)i * 4;
+        o[0] = hh.t;
+        std::memcpy(&o[1], &hh.prim, 4);
+        o[2] = hh.u;
+        o[3] = hh.v;
+        if (visits) visits[i] = nv;
+        if (tests) tests[i] = nt;
+    }
+}
+void oracle_occluded_batch(void* h, const float* rays, uint32_t n, uint32_t* out) {
+    const Scene& s = *static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + (size_t)i * 8;
+        out[i] = occluded_f32(s.bvh, r, r + 4, r[3], r[7]) ? 1u : 0u;
+    }
+}
+// FP64 brute force; rays given as floats (exactly representable in double).
+void oracle_brute_batch(void* h, const float* rays, uint32_t n, double rel_tie, double* t_out,
+                        uint32_t* prim_out, uint32_t* nties_out, uint32_t* ties_out, uint32_t max_ties) {
+    const Scene& s = *static_cast<Scene*>(h);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + (size_t)i * 8;
+        const double o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
+        const HitF64 hh = brute_closest_f64(s.pos.data(), s.n, o, d, r[3], r[7], rel_tie,
+                                            ties_out ? ties_out + (size_t)i * max_ties : nullptr, max_ties);
+        t_out[i] = hh.t;
+        prim_out[i] = hh.prim;
+        if (nties_out) nties_out[i] = hh.n_ties;
+    }
+}
+
+void oracle_render(void* h, uint32_t spp, uint32_t first_sample, int precision, int threads,
+                   double* sum, double* sumsq, uint32_t row_begin, uint32_t row_end) {
+    render(*static_cast<Scene*>(h), spp, first_sample, precision, threads, sum, sumsq, row_begin, row_end);
+}
+void oracle_path(void* h, uint32_t px, uint32_t py, uint32_t sidx, int precision, double* out3) {
+    trace_path(*static_cast<Scene*>(h), px, py, sidx, precision, out3);
+}
+
+}  // extern "C"

@@ -1,0 +1,627 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY (see oracle_core.hpp / oracle_render.hpp).
+#include "oracle_render.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <thread>
+
+namespace oracle {
+
+namespace {
+
+template <class R> struct V3 {
+    R x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(R a, R b, R c) : x(a), y(b), z(c) {}
+    R operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, V3<R> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> inline V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> inline V3<R> normalize(V3<R> a) { return a / length(a); }
+template <class R> inline R maxc(V3<R> a) { return std::max(a.x, std::max(a.y, a.z)); }
+template <class R> inline R lum(V3<R> a) { return R(0.2126) * a.x + R(0.7152) * a.y + R(0.0722) * a.z; }
+template <class R> inline V3<R> v3(const float* p) { return {R(p[0]), R(p[1]), R(p[2])}; }
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- textures
+// Bilinear, REPEAT in both axes (the reference's bindless sampler), texel
+// centres at (i + 0.5) / size, row 0 at v = 0.
+template <class R>
+void tex_bilinear(const Texture& t, R u, R v, R out[4]) {
+    const R x = u * R(t.w) - R(0.5), y = v * R(t.h) - R(0.5);
+    const R fx0 = std::floor(x), fy0 = std::floor(y);
+    const R fx = x - fx0, fy = y - fy0;
+    auto wrap = [](long long i, uint32_t n) { long long m = i % (long long)n; return (uint32_t)(m < 0 ? m + n : m); };
+    const uint32_t x0 = wrap((long long)fx0, t.w), x1 = wrap((long long)fx0 + 1, t.w);
+    const uint32_t y0 = wrap((long long)fy0, t.h), y1 = wrap((long long)fy0 + 1, t.h);
+    for (uint32_t c = 0; c < 4; ++c) out[c] = 0;
+    for (uint32_t c = 0; c < t.c && c < 4; ++c) {
+        const R a = t.data[((size_t)y0 * t.w + x0) * t.c + c], b = t.data[((size_t)y0 * t.w + x1) * t.c + c];
+        const R d = t.data[((size_t)y1 * t.w + x0) * t.c + c], e = t.data[((size_t)y1 * t.w + x1) * t.c + c];
+        out[c] = (a * (R(1) - fx) + b * fx) * (R(1) - fy) + (d * (R(1) - fx) + e * fx) * fy;
+    }
+}
+
+// ---------------------------------------------------------------- env
+template <class R>
+V3<R> env_eval(const Scene& sc, V3<R> d) {
+    if (sc.env_w == 0) return {};
+    const R th = std::acos(std::min(R(1), std::max(R(-1), d.y)));
+    const R ph = std::atan2(d.z, d.x);
+    const R u = ph / R(2 * kPi) + R(0.5), v = th / R(kPi);
+    const R x = u * R(sc.env_w) - R(0.5), y = v * R(sc.env_h) - R(0.5);
+    const R fx0 = std::floor(x), fy0 = std::floor(y);
+    const R fx = x - fx0, fy = y - fy0;
+    auto wrapx = [&](long long i) { long long m = i % (long long)sc.env_w; return (uint32_t)(m < 0 ? m + sc.env_w : m); };
+    auto clampy = [&](long long i) { return (uint32_t)std::min<long long>(std::max<long long>(i, 0), sc.env_h - 1); };
+    const uint32_t x0 = wrapx((long long)fx0), x1 = wrapx((long long)fx0 + 1);
+    const uint32_t y0 = clampy((long long)fy0), y1 = clampy((long long)fy0 + 1);
+    R out[3];
+    for (int c = 0; c < 3; ++c) {
+        const R a = sc.env[((size_t)y0 * sc.env_w + x0) * 4 + c], b = sc.env[((size_t)y0 * sc.env_w + x1) * 4 + c];
+        const R e = sc.env[((size_t)y1 * sc.env_w + x0) * 4 + c], f = sc.env[((size_t)y1 * sc.env_w + x1) * 4 + c];
+        out[c] = (a * (R(1) - fx) + b * fx) * (R(1) - fy) + (e * (R(1) - fx) + f * fx) * fy;
+    }
+    const R s = R(sc.env_intensity);
+    return {out[0] * s, out[1] * s, out[2] * s};
+}
+
+// lower_bound search of searchMarginal/searchConditional (env_sampling.glsl:16-37)
+inline uint32_t cdf_search(const float* cdf, uint32_t n, float u) {
+    int lo = 0, hi = (int)n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (cdf[mid] < u) lo = mid + 1; else hi = mid;
+    }
+    return (uint32_t)lo;
+}
+
+template <class R>
+bool env_sample(const Scene& sc, R u1, R u2, V3<R>& dir, R& pdf) {
+    const uint32_t W = sc.env_w, H = sc.env_h;
+    const uint32_t y = cdf_search(sc.env_marg.data(), H, (float)u1);
+    const float* row = sc.env_cond.data() + (size_t)y * W;
+    const uint32_t x = cdf_search(row, W, (float)u2);
+    const R m0 = y > 0 ? R(sc.env_marg[y - 1]) : R(0), m1 = R(sc.env_marg[y]);
+    const R c0 = x > 0 ? R(row[x - 1]) : R(0), c1 = R(row[x]);
+    const R dm = m1 - m0, dc = c1 - c0;
+    if (!(dm > 0) || !(dc > 0)) { pdf = 0; return false; }
+    R fy = (u1 - m0) / dm, fx = (u2 - c0) / dc;
+    fy = std::min(std::max(fy, R(0)), R(0.99999994));
+    fx = std::min(std::max(fx, R(0)), R(0.99999994));
+    const R v = (R(y) + fy) / R(H), u = (R(x) + fx) / R(W);
+    const R th = v * R(kPi), ph = (u - R(0.5)) * R(2 * kPi);
+    const R st = std::sin(th);
+    dir = V3<R>(st * std::cos(ph), std::cos(th), st * std::sin(ph));
+    if (!(st > 0)) { pdf = 0; return false; }
+    pdf = dm * dc * R(W) * R(H) / (R(2 * kPi * kPi) * st);
+    return pdf > 0;
+}
+
+template <class R>
+R env_pdf(const Scene& sc, V3<R> d) {
+    const uint32_t W = sc.env_w, H = sc.env_h;
+    const R th = std::acos(std::min(R(1), std::max(R(-1), d.y)));
+    const R ph = std::atan2(d.z, d.x);
+    const R u = ph / R(2 * kPi) + R(0.5), v = th / R(kPi);
+    const long long xi = std::min<long long>(std::max<long long>((long long)(u * R(W)), 0), W - 1);
+    const long long yi = std::min<long long>(std::max<long long>((long long)(v * R(H)), 0), H - 1);
+    const float* row = sc.env_cond.data() + (size_t)yi * W;
+    const R dc = R(row[xi]) - (xi > 0 ? R(row[xi - 1]) : R(0));
+    const R dm = R(sc.env_marg[yi]) - (yi > 0 ? R(sc.env_marg[yi - 1]) : R(0));
+    const R st = std::sin(th);
+    if (!(st > 0)) return 0;
+    return std::max(R(0), dm * dc * R(W) * R(H) / (R(2 * kPi * kPi) * st));
+}
+
+// ---------------------------------------------------------------- BSDF
+template <class R> struct Frame {
+    V3<R> t, b, n;
+    explicit Frame(V3<R> nn) : n(nn) {   // Duff et al. 2017, "Building an orthonormal basis, revisited"
+        const R sign = std::copysign(R(1), n.z);
+        const R a = R(-1) / (sign + n.z);
+        const R bb = n.x * n.y * a;
+        t = V3<R>(R(1) + sign * n.x * n.x * a, sign * bb, -sign * n.x);
+        b = V3<R>(bb, sign + n.y * n.y * a, -n.y);
+    }
+    V3<R> to_local(V3<R> v) const { return {dot(v, t), dot(v, b), dot(v, n)}; }
+    V3<R> to_world(V3<R> v) const { return t * v.x + b * v.y + n * v.z; }
+};
+
+template <class R> struct Bsdf {
+    V3<R> albedo;
+    R rough, metal, f0d, alpha;
+};
+
+template <class R> inline R smith_lambda(R c, R a) {
+    const R c2 = c * c;
+    const R t2 = std::max(R(0), R(1) - c2) / c2;
+    return R(0.5) * (R(-1) + std::sqrt(R(1) + a * a * t2));
+}
+
+template <class R> inline R ggx_d(R ch, R a) {
+    const R a2 = a * a;
+    const R den = ch * ch * (a2 - R(1)) + R(1);
+    return a2 / (R(kPi) * den * den);
+}
+
+template <class R> inline R lobe_prob(const Bsdf<R>& m, R cos_o) {
+    R fd = 0;
+    if (m.f0d > 0) {
+        const R q = R(1) - cos_o;
+        const R q5 = q * q * q * q * q;
+        fd = std::min(R(1), std::max(R(0.25), m.f0d + (R(1) - m.f0d) * q5));
+    }
+    return m.metal + (R(1) - m.metal) * fd;
+}
+
+template <class R>
+V3<R> bsdf_eval(const Bsdf<R>& m, V3<R> ns, V3<R> wo, V3<R> wi) {
+    const R co = dot(ns, wo), ci = dot(ns, wi);
+    if (!(co > 0) || !(ci > 0)) return {};
+    const V3<R> h = normalize(wo + wi);
+    const R ch = std::max(R(0), dot(ns, h));
+    const R voh = std::min(R(1), std::max(R(0), dot(wo, h)));
+    const R D = ggx_d(ch, m.alpha);
+    const R G2 = R(1) / (R(1) + smith_lambda(co, m.alpha) + smith_lambda(ci, m.alpha));
+    const R q = R(1) - voh;
+    const R s = q * q * q * q * q;
+    const V3<R> F0(m.f0d + (m.albedo.x - m.f0d) * m.metal, m.f0d + (m.albedo.y - m.f0d) * m.metal,
+                   m.f0d + (m.albedo.z - m.f0d) * m.metal);
+    // Schlick with F90 = 1 for any specular surface, 0 for a pure Lambert one
+    const R F90 = (m.f0d > 0 || m.metal > 0) ? R(1) : R(0);
+    const V3<R> F(F0.x + (F90 - F0.x) * s, F0.y + (F90 - F0.y) * s, F0.z + (F90 - F0.z) * s);
+    const R sp = D * G2 / (R(4) * co * ci);
+    const R kd = (R(1) - m.metal) / R(kPi);
+    return V3<R>(F.x * sp + (R(1) - F.x) * kd * m.albedo.x, F.y * sp + (R(1) - F.y) * kd * m.albedo.y,
+                 F.z * sp + (R(1) - F.z) * kd * m.albedo.z);
+}
+
+template <class R>
+R bsdf_pdf(const Bsdf<R>& m, V3<R> ns, V3<R> wo, V3<R> wi) {
+    const R co = dot(ns, wo), ci = dot(ns, wi);
+    if (!(co > 0) || !(ci > 0)) return 0;
+    const R ps = lobe_prob(m, co);
+    R pspec = 0;
+    if (ps > 0) {
+        const V3<R> h = normalize(wo + wi);
+        const R ch = std::max(R(0), dot(ns, h));
+        const R G1 = R(1) / (R(1) + smith_lambda(co, m.alpha));
+        pspec = G1 * ggx_d(ch, m.alpha) / (R(4) * co);
+    }
+    return ps * pspec + (R(1) - ps) * ci / R(kPi);
+}
+
+// Heitz 2018, "Sampling the GGX Distribution of Visible Normals", isotropic.
+template <class R>
+V3<R> sample_vndf(V3<R> Ve, R a, R u1, R u2) {
+    const V3<R> Vh = normalize(V3<R>(a * Ve.x, a * Ve.y, Ve.z));
+    const R lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+    const V3<R> T1 = lensq > 0 ? V3<R>(-Vh.y, Vh.x, 0) / std::sqrt(lensq) : V3<R>(1, 0, 0);
+    const V3<R> T2 = cross(Vh, T1);
+    const R r = std::sqrt(u1), phi = R(2 * kPi) * u2;
+    const R t1 = r * std::cos(phi);
+    R t2 = r * std::sin(phi);
+    const R s = R(0.5) * (R(1) + Vh.z);
+    t2 = (R(1) - s) * std::sqrt(std::max(R(0), R(1) - t1 * t1)) + s * t2;
+    const V3<R> Nh = T1 * t1 + T2 * t2 + Vh * std::sqrt(std::max(R(0), R(1) - t1 * t1 - t2 * t2));
+    return normalize(V3<R>(a * Nh.x, a * Nh.y, std::max(R(0), Nh.z)));
+}
+
+template <class R>
+bool bsdf_sample(const Bsdf<R>& m, V3<R> ns, V3<R> wo, R ul, R u1, R u2, V3<R>& wi) {
+    const Frame<R> fr(ns);
+    const V3<R> wl = fr.to_local(wo);
+    if (!(wl.z > 0)) return false;
+    const R ps = lobe_prob(m, wl.z);
+    V3<R> il;
+    if (ul < ps) {
+        const V3<R> hl = sample_vndf(wl, m.alpha, u1, u2);
+        const R d = dot(wl, hl);
+        il = hl * (R(2) * d) - wl;
+    } else {
+        const R r = std::sqrt(u1), phi = R(2 * kPi) * u2;
+        il = V3<R>(r * std::cos(phi), r * std::sin(phi), std::sqrt(std::max(R(0), R(1) - u1)));
+    }
+    if (!(il.z > 0)) return false;
+    wi = normalize(fr.to_world(il));
+    return true;
+}
+
+// ---------------------------------------------------------------- geometry access
+template <class R> struct Hit {
+    bool hit = false;
+    uint32_t prim = 0;
+    R t = 0, u = 0, v = 0;
+};
+
+template <class R>
+Hit<R> trace_closest(const Scene& sc, V3<R> o, V3<R> d, R tmax) {
+    Hit<R> h;
+    if constexpr (sizeof(R) == 4) {
+        const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+        const HitF32 r = trace_f32(sc.bvh, of, df, 0.0f, (float)tmax, nullptr, nullptr);
+        if (r.prim != 0xFFFFFFFFu) { h.hit = true; h.prim = r.prim; h.t = r.t; h.u = r.u; h.v = r.v; }
+    } else {
+        const double od[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+        if (sc.n <= 64) {
+            const HitF64 r = brute_closest_f64(sc.pos.data(), sc.n, od, dd, 0.0, tmax, 0.0, nullptr, 0);
+            if (r.prim != 0xFFFFFFFFu) { h.hit = true; h.prim = r.prim; h.t = r.t; h.u = r.u; h.v = r.v; }
+        } else {
+            // FP32 LBVH traversal for candidate pruning, then FP64 refinement of the
+            // winning triangle and every triangle within a relative 1e-4 band.
+            const float of[3] = {(float)o.x, (float)o.y, (float)o.z}, df[3] = {(float)d.x, (float)d.y, (float)d.z};
+            const HitF32 r = trace_f32(sc.bvh, of, df, 0.0f, (float)std::min(tmax, 3.0e38), nullptr, nullptr);
+            if (r.prim != 0xFFFFFFFFu) {
+                const HitF64 b = brute_closest_f64(sc.pos.data() + (size_t)r.prim * 9, 1, od, dd, 0.0,
+                                                   tmax, 0.0, nullptr, 0);
+                h.hit = true; h.prim = r.prim;
+                if (b.prim != 0xFFFFFFFFu) { h.t = b.t; h.u = b.u; h.v = b.v; }
+                else { h.t = r.t; h.u = r.u; h.v = r.v; }
+            }
+        }
+    }
+    return h;
+}
+
+template <class R>
+bool occluded(const Scene& sc, V3<R> o, V3<R> d, R tmax) {
+    if constexpr (sizeof(R) == 4) {
+        const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+        return occluded_f32(sc.bvh, of, df, 0.0f, tmax);
+    } else {
+        if (sc.n <= 64) {
+            const double od[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+            const HitF64 r = brute_closest_f64(sc.pos.data(), sc.n, od, dd, 0.0, tmax, 0.0, nullptr, 0);
+            return r.prim != 0xFFFFFFFFu;
+        }
+        const float of[3] = {(float)o.x, (float)o.y, (float)o.z}, df[3] = {(float)d.x, (float)d.y, (float)d.z};
+        return occluded_f32(sc.bvh, of, df, 0.0f, (float)std::min(tmax, 3.0e38));
+    }
+}
+
+template <class R> struct Surf {
+    V3<R> p, ng, ns;           // ng, ns face the incoming ray
+    R u = 0, v = 0;            // texture coordinates
+    bool front = true;
+    uint32_t mat = 0;
+};
+
+template <class R>
+Surf<R> interact(const Scene& sc, const Hit<R>& h, V3<R> d) {
+    Surf<R> s;
+    const float* tp = &sc.pos[(size_t)h.prim * 9];
+    const V3<R> v0 = v3<R>(tp), v1 = v3<R>(tp + 3), v2 = v3<R>(tp + 6);
+    const R w = R(1) - h.u - h.v;
+    s.p = v0 * w + v1 * h.u + v2 * h.v;
+    V3<R> ng = normalize(cross(v1 - v0, v2 - v0));
+    V3<R> ns = ng;
+    if (!sc.nrm.empty()) {
+        const float* np = &sc.nrm[(size_t)h.prim * 9];
+        const V3<R> n = v3<R>(np) * w + v3<R>(np + 3) * h.u + v3<R>(np + 6) * h.v;
+        if (dot(n, n) > R(1e-12)) ns = normalize(n);
+    }
+    if (!sc.uv.empty()) {
+        const float* up = &sc.uv[(size_t)h.prim * 6];
+        s.u = R(up[0]) * w + R(up[2]) * h.u + R(up[4]) * h.v;
+        s.v = R(up[1]) * w + R(up[3]) * h.u + R(up[5]) * h.v;
+    }
+    s.front = dot(ng, d) < 0;
+    if (!s.front) { ng = -ng; ns = -ns; }
+    const V3<R> wo = -d;
+    if (!(dot(ns, ng) > 0) || !(dot(ns, wo) > R(1e-6))) ns = ng;
+    s.ng = ng; s.ns = ns;
+    s.mat = sc.mat[h.prim];
+    return s;
+}
+
+template <class R>
+Bsdf<R> material_at(const Scene& sc, const std::vector<Texture>& tex, const Surf<R>& s) {
+    const ohpt_material& m = sc.materials[s.mat];
+    Bsdf<R> b;
+    b.albedo = V3<R>(m.base_color[0], m.base_color[1], m.base_color[2]);
+    b.rough = m.roughness;
+    b.metal = m.metallic;
+    b.f0d = m.specular_f0;
+    R t4[4];
+    if (m.albedo_tex >= 0) {
+        tex_bilinear(tex[(size_t)m.albedo_tex], s.u, s.v, t4);
+        b.albedo = V3<R>(b.albedo.x * t4[0], b.albedo.y * t4[1], b.albedo.z * t4[2]);
+    }
+    if (m.roughness_tex >= 0) { tex_bilinear(tex[(size_t)m.roughness_tex], s.u, s.v, t4); b.rough *= t4[0]; }
+    if (m.metallic_tex >= 0) { tex_bilinear(tex[(size_t)m.metallic_tex], s.u, s.v, t4); b.metal *= t4[0]; }
+    b.rough = std::min(R(1), std::max(R(0.04), b.rough));
+    b.metal = std::min(R(1), std::max(R(0), b.metal));
+    b.alpha = b.rough * b.rough;
+    return b;
+}
+
+template <class R> inline R balance(R a, R b) { return a / std::max(a + b, R(1e-30)); }
+
+template <class R> inline R offset_eps(const Scene& sc, V3<R> p) {
+    const R e = sc.st.ray_eps > 0 ? R(sc.st.ray_eps) : R(1e-4);
+    return e * std::max(R(1), std::max(std::fabs(p.x), std::max(std::fabs(p.y), std::fabs(p.z))));
+}
+
+// Reference analytic light sample (pt_raygen_offline.rgen:311-389). Returns
+// the direction, distance and the reference's Le*weight product.
+template <class R>
+bool sample_analytic(const ohpt_light& L, V3<R> p, R u1, R u2, V3<R>& wi, R& dist, V3<R>& Lw) {
+    const R type = L.position_type[3];
+    const V3<R> c = v3<R>(L.position_type);
+    const V3<R> col = v3<R>(L.color_intensity);
+    const R inten = L.color_intensity[3];
+    const R rad = std::max(R(L.dir_param[3]), R(0.01));
+    const R area4 = R(4.0) * R(3.14159) * rad * rad;
+    const V3<R> Le = col * (inten / std::max(area4, R(0.01)));
+    R weight = 0;
+    if (type < R(0.5) || (type >= R(1.5) && type < R(2.5))) {
+        const R ct = R(1) - R(2) * u1;
+        const R st = std::sqrt(std::max(R(0), R(1) - ct * ct));
+        const R ph = R(6.2831853) * u2;
+        const V3<R> off = V3<R>(st * std::cos(ph), st * std::sin(ph), ct) * rad;
+        const V3<R> lp = c + off;
+        const V3<R> toL = lp - p;
+        dist = length(toL);
+        if (!(dist > 0)) return false;
+        wi = toL / dist;
+        if (type < R(0.5)) {
+            const R lc = std::max(dot(-wi, normalize(off)), R(0));
+            weight = lc * area4 / (dist * dist);
+        } else {
+            const V3<R> ld = normalize(v3<R>(L.dir_param));
+            const R ca = dot(-wi, ld);
+            const R ic = std::cos(R(L.dir_param[3]) * R(kPi / 180.0));
+            const R oc = std::cos(R(L.extra[3]) * R(kPi / 180.0));
+            R sf = std::min(R(1), std::max(R(0), (ca - oc) / (ic - oc + R(0.001))));
+            sf *= sf;
+            weight = area4 / (dist * dist) * sf;
+        }
+    } else if (type < R(1.5)) {
+        wi = -normalize(v3<R>(L.dir_param));
+        dist = R(INFINITY);
+        weight = 1;
+    } else {
+        const V3<R> e1 = v3<R>(L.extra), e2 = v3<R>(L.extra2);
+        const V3<R> lp = c + e1 * u1 + e2 * u2;
+        const V3<R> nl = normalize(cross(e1, e2));
+        const V3<R> toL = lp - p;
+        dist = length(toL);
+        if (!(dist > 0)) return false;
+        wi = toL / dist;
+        const R lc = std::max(dot(-wi, nl), R(0));
+        weight = lc * R(L.extra2[3]) / (dist * dist);
+    }
+    Lw = Le * weight;
+    return weight > 0;
+}
+
+template <class R>
+V3<R> path_sample(const Scene& sc, uint32_t px, uint32_t py, uint32_t sidx) {
+    const ohpt_settings& st = sc.st;
+    const std::vector<Texture>& tex_s = sc.textures;
+    const std::vector<Texture>& tex_e = sc.textures_eval.empty() ? sc.textures : sc.textures_eval;
+    const bool light_nee = !(st.flags & OHPT_FLAG_NO_LIGHT_NEE);
+    const bool env_nee = !(st.flags & OHPT_FLAG_NO_ENV_NEE) && sc.env_w > 0;
+    const uint32_t NA = (uint32_t)sc.lights.size();
+    const uint32_t NL = NA + (sc.emit_total > 0 ? 1u : 0u);
+
+    float jit[2];
+    draw_camera(st.sampler, px, py, sc.width, sidx, jit);
+    const R jx = R(jit[0]) - R(0.5), jy = R(jit[1]) - R(0.5);
+    const R ndcx = (R(px) + R(0.5) + jx) / R(sc.width) * R(2) - R(1);
+    const R ndcy = (R(py) + R(0.5) + jy) / R(sc.height) * R(2) - R(1);
+    const R ty = R(sc.cam.tan_half_fov_y), tx = ty * (R(sc.width) / R(sc.height));
+    const V3<R> fwd = v3<R>(sc.cam.forward), right = v3<R>(sc.cam.right), up = v3<R>(sc.cam.up);
+    V3<R> d = normalize(fwd + right * (ndcx * tx) - up * (ndcy * ty));
+    V3<R> o = v3<R>(sc.cam.position);
+
+    V3<R> T(1, 1, 1), Ts(1, 1, 1), L;
+    R prev_pdf = 0;
+    for (uint32_t b = 0; b <= st.max_bounces; ++b) {
+        const Hit<R> h = trace_closest(sc, o, d, R(INFINITY));
+        if (!h.hit) {
+            if (sc.env_w > 0) {
+                const V3<R> Le = env_eval(sc, d);
+                const R w = (b == 0 || !env_nee) ? R(1) : balance(prev_pdf, env_pdf(sc, d));
+                L = L + T * Le * w;
+            }
+            break;
+        }
+        const Surf<R> s = interact(sc, h, d);
+        const ohpt_material& mr = sc.materials[s.mat];
+        const V3<R> Le(mr.emissive[0], mr.emissive[1], mr.emissive[2]);
+        if (s.front && maxc(Le) > 0) {
+            R w = 1;
+            if (b > 0 && light_nee && NL > 0 && sc.emit_total > 0) {
+                const R cl = -dot(s.ng, d);   // ng faces the ray, so this is the emitter cosine
+                const R pa = lum(Le) / (R(NL) * R(sc.emit_total));
+                const R pl = pa * h.t * h.t / std::max(cl, R(1e-30));
+                w = balance(prev_pdf, pl);
+            }
+            L = L + T * Le * w;
+        }
+        if (b == st.max_bounces) break;
+
+        VertexSamples vs;
+        draw_vertex(st.sampler, px, py, sc.width, sidx, b, vs);
+        const Bsdf<R> me = material_at(sc, tex_e, s);
+        const Bsdf<R> ms = material_at(sc, tex_s, s);
+        const V3<R> wo = -d;
+        const R eps = offset_eps(sc, s.p);
+        const V3<R> po = s.p + s.ng * eps;
+
+        // --- next-event estimation: one light
+        if (light_nee && NL > 0) {
+            uint32_t k = (uint32_t)(R(vs.u[0]) * R(NL));
+            if (k > NL - 1) k = NL - 1;
+            if (k < NA) {
+                V3<R> wi, Lw; R dist;
+                if (sample_analytic(sc.lights[k], s.p, R(vs.u[1]), R(vs.u[2]), wi, dist, Lw)) {
+                    const R ci = dot(s.ns, wi);
+                    if (ci > 0 && dot(s.ng, wi) > 0) {
+                        const V3<R> f = bsdf_eval(me, s.ns, wo, wi);
+                        if (maxc(f) > 0) {
+                            const R tmaxs = std::isinf(dist) ? R(INFINITY) : dist * R(1 - 1e-4) - eps;
+                            if (!occluded(sc, po, wi, tmaxs)) L = L + T * f * Lw * (ci * R(NL));
+                        }
+                    }
+                }
+            } else {
+                R ut = R(vs.u[0]) * R(NL) - R(k);
+                ut = std::min(std::max(ut, R(0)), R(0.99999994));
+                const auto it = std::lower_bound(sc.emit_cdf.begin(), sc.emit_cdf.end(), (double)ut,
+                                                 [](double a, double x) { return a <= x; });
+                const size_t ei = std::min<size_t>((size_t)(it - sc.emit_cdf.begin()), sc.emit_tri.size() - 1);
+                const uint32_t et = sc.emit_tri[ei];
+                const float* tp = &sc.pos[(size_t)et * 9];
+                const V3<R> a0 = v3<R>(tp), a1 = v3<R>(tp + 3), a2 = v3<R>(tp + 6);
+                const R su = std::sqrt(R(vs.u[1]));
+                const R b0 = R(1) - su, b1 = R(vs.u[2]) * su, b2 = R(1) - b0 - b1;
+                const V3<R> x = a0 * b0 + a1 * b1 + a2 * b2;
+                const V3<R> cr = cross(a1 - a0, a2 - a0);
+                const R area = R(0.5) * length(cr);
+                const V3<R> nl = normalize(cr);
+                const V3<R> toL = x - s.p;
+                const R dist = length(toL);
+                if (dist > 0 && area > 0) {
+                    const V3<R> wi = toL / dist;
+                    const R cl = -dot(nl, wi);
+                    const R ci = dot(s.ns, wi);
+                    if (cl > 0 && ci > 0 && dot(s.ng, wi) > 0) {
+                        const ohpt_material& em = sc.materials[sc.mat[et]];
+                        const V3<R> Le2(em.emissive[0], em.emissive[1], em.emissive[2]);
+                        const R pa = lum(Le2) / (R(NL) * R(sc.emit_total));
+                        const R pl = pa * dist * dist / cl;
+                        const V3<R> f = bsdf_eval(me, s.ns, wo, wi);
+                        if (maxc(f) > 0 && pl > 0) {
+                            const R w = balance(pl, bsdf_pdf(ms, s.ns, wo, wi));
+                            if (!occluded(sc, po, wi, dist * R(1 - 1e-4) - eps))
+                                L = L + T * f * Le2 * (ci * w / pl);
+                        }
+                    }
+                }
+            }
+        }
+        // --- next-event estimation: environment
+        if (env_nee) {
+            V3<R> wi; R pe;
+            if (env_sample(sc, R(vs.u[3]), R(vs.u[4]), wi, pe)) {
+                const R ci = dot(s.ns, wi);
+                if (ci > 0 && dot(s.ng, wi) > 0) {
+                    const V3<R> f = bsdf_eval(me, s.ns, wo, wi);
+                    if (maxc(f) > 0 && !occluded(sc, po, wi, R(INFINITY))) {
+                        const R w = balance(pe, bsdf_pdf(ms, s.ns, wo, wi));
+                        L = L + T * f * env_eval(sc, wi) * (ci * w / pe);
+                    }
+                }
+            }
+        }
+        // --- Russian roulette
+        if (b >= st.rr_start) {
+            const R q = std::min(R(1), maxc(Ts));
+            if (!(R(vs.u[5]) < q)) break;
+            T = T / q; Ts = Ts / q;
+        }
+        // --- BSDF sampling
+        V3<R> wi;
+        if (!bsdf_sample(ms, s.ns, wo, R(vs.u[6]), R(vs.u[7]), R(vs.u[8]), wi)) break;
+        const R ci = dot(s.ns, wi);
+        if (!(ci > 0) || !(dot(s.ng, wi) > 0)) break;
+        const R pdf = bsdf_pdf(ms, s.ns, wo, wi);
+        if (!(pdf > 0)) break;
+        const V3<R> fe = bsdf_eval(me, s.ns, wo, wi);
+        const V3<R> fs = bsdf_eval(ms, s.ns, wo, wi);
+        T = T * fe * (ci / pdf);
+        Ts = Ts * fs * (ci / pdf);
+        if (!(maxc(Ts) > 0) && !(maxc(T) > 0)) break;
+        prev_pdf = pdf;
+        o = po;
+        d = wi;
+    }
+    if ((st.flags & OHPT_FLAG_FIREFLY_CLAMP) && st.firefly_clamp > 0) {
+        const R l = lum(L);
+        if (l > R(st.firefly_clamp)) L = L * (R(st.firefly_clamp) / l);
+    }
+    return L;
+}
+
+}  // namespace
+
+void Scene::build_emitters() {
+    emit_tri.clear();
+    emit_cdf.clear();
+    emit_total = 0.0;
+    std::vector<double> w;
+    for (uint32_t i = 0; i < n; ++i) {
+        const ohpt_material& m = materials[mat[i]];
+        const double l = 0.2126 * m.emissive[0] + 0.7152 * m.emissive[1] + 0.0722 * m.emissive[2];
+        if (!(l > 0)) continue;
+        const float* t = &pos[(size_t)i * 9];
+        const V3<double> a0 = v3<double>(t), a1 = v3<double>(t + 3), a2 = v3<double>(t + 6);
+        const double area = 0.5 * length(cross(a1 - a0, a2 - a0));
+        if (!(area > 0)) continue;
+        emit_tri.push_back(i);
+        w.push_back(area * l);
+        emit_total += area * l;
+    }
+    double acc = 0;
+    for (double x : w) { acc += x; emit_cdf.push_back(acc / emit_total); }
+    if (!emit_cdf.empty()) emit_cdf.back() = 1.0;
+}
+
+void Scene::commit() {
+    lbvh_build(pos.data(), n, bvh);
+    bvh_ready = true;
+    build_emitters();
+    if (env_w > 0) env_cdf_build(env.data(), (int)env_w, (int)env_h, env_marg, env_cond, env_integral);
+}
+
+void trace_path(const Scene& sc, uint32_t px, uint32_t py, uint32_t sidx, int precision, double out[3]) {
+    if (precision == 64) {
+        const V3<double> L = path_sample<double>(sc, px, py, sidx);
+        out[0] = L.x; out[1] = L.y; out[2] = L.z;
+    } else {
+        const V3<float> L = path_sample<float>(sc, px, py, sidx);
+        out[0] = L.x; out[1] = L.y; out[2] = L.z;
+    }
+}
+
+void render(const Scene& sc, uint32_t spp, uint32_t first_sample, int precision, int threads,
+            double* sum, double* sumsq, uint32_t row_begin, uint32_t row_end) {
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    row_end = std::min(row_end, sc.height);
+    std::atomic<uint32_t> next{row_begin};
+    auto work = [&]() {
+        for (;;) {
+            const uint32_t y = next.fetch_add(1);
+            if (y >= row_end) break;
+            for (uint32_t x = 0; x < sc.width; ++x) {
+                double s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+                for (uint32_t k = 0; k < spp; ++k) {
+                    double L[3];
+                    trace_path(sc, x, y, first_sample + k, precision, L);
+                    for (int c = 0; c < 3; ++c) { s[c] += L[c]; q[c] += L[c] * L[c]; }
+                }
+                const size_t o = ((size_t)y * sc.width + x) * 3;
+                for (int c = 0; c < 3; ++c) { sum[o + c] += s[c]; sumsq[o + c] += q[c]; }
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+}
+
+}  // namespace oracle

@@ -1,0 +1,64 @@
+// ORACLE — TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
+//
+// Path-integrator restatement. Estimator (the same one the HIP wavefront
+// kernels implement):
+//   * pinhole camera of pt_raygen_offline.rgen:144-162;
+//   * vertices v_0..v_D (D = max_bounces): emission at v_b (MIS-weighted
+//     against light sampling from v_{b-1}), then for b < D next-event
+//     estimation to one light (uniform pick over analytic lights + the
+//     emitter-triangle group, reference light formulas of
+//     pt_raygen_offline.rgen:311-389) and one environment sample
+//     (sampleEnvMap of env_sampling.glsl:52-74, continuous inside the texel),
+//     both balance-heuristic MIS (mis.glsl:7-10) against the BSDF pdf;
+//   * Russian roulette from vertex rr_start with p = min(1, max(throughput));
+//   * BSDF: Lambert + GGX (Smith height-correlated G2, Schlick F) with lobe
+//     pick, VNDF sampling (Heitz 2018, ggx_aniso.glsl:68-126) and cosine
+//     sampling; mixture pdf.
+#pragma once
+
+#include "oracle_core.hpp"
+
+namespace oracle {
+
+struct Texture {
+    uint32_t w = 0, h = 0, c = 0;
+    std::vector<float> data;
+};
+
+struct Scene {
+    uint32_t n = 0;
+    std::vector<float> pos, nrm, uv;     // n*9, n*9 (opt), n*6 (opt)
+    std::vector<uint32_t> mat;           // n
+    std::vector<ohpt_material> materials;
+    std::vector<Texture> textures;       // sampling/pdf side
+    std::vector<Texture> textures_eval;  // evaluation side (empty = same)
+    std::vector<ohpt_light> lights;
+    uint32_t env_w = 0, env_h = 0;
+    std::vector<float> env;              // RGBA32F
+    float env_intensity = 1.0f;
+    std::vector<float> env_marg, env_cond;
+    float env_integral = 0.0f;
+    // emitter triangles
+    std::vector<uint32_t> emit_tri;
+    std::vector<double> emit_cdf;        // normalised, size = emit_tri.size()
+    double emit_total = 0.0;             // sum of area * lum(Le)
+    ohpt_camera cam{};
+    uint32_t width = 0, height = 0;
+    ohpt_settings st{};
+    Bvh bvh;
+    bool bvh_ready = false;
+
+    void commit();                       // builds BVH + emitter table + env CDF
+    void build_emitters();
+};
+
+// Render `spp` samples per pixel with sample indices first_sample .. first_sample+spp-1.
+// precision 32 or 64. Outputs per-pixel RGB sum and sum of squares (double, W*H*3).
+void render(const Scene& sc, uint32_t spp, uint32_t first_sample, int precision, int threads,
+            double* sum, double* sumsq, uint32_t row_begin, uint32_t row_end);
+
+// One path sample (for unit tests).
+void trace_path(const Scene& sc, uint32_t px, uint32_t py, uint32_t sample_index, int precision,
+                double out[3]);
+
+}  // namespace oracle

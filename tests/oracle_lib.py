@@ -1,0 +1,224 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) and of the reference
+C++ built into oracle/_ref. Test infrastructure: only tests/, smoke() and the
+bench's cpu_baseline leg use it, as the checker."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+LIB_PATH = ORACLE_DIR / "liboracle.so"
+REF_PATH = ORACLE_DIR / "_ref" / "libohao_ref.so"
+
+_lib = None
+_ref = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        vp, f32p, u32p, f64p, u64p = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32), \
+            C.POINTER(C.c_double), C.POINTER(C.c_uint64)
+        L.oracle_pcg32_seq.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, u32p]
+        for n in ("oracle_sobol_int", "oracle_owen", "oracle_hash_pixel"):
+            getattr(L, n).argtypes = [C.c_uint32, C.c_uint32]
+            getattr(L, n).restype = C.c_uint32
+        L.oracle_sobol_sample.argtypes = [C.c_uint32] * 4
+        L.oracle_sobol_sample.restype = C.c_float
+        L.oracle_sobol_unscrambled.argtypes = [C.c_uint32] * 2
+        L.oracle_sobol_unscrambled.restype = C.c_float
+        L.oracle_draw_camera.argtypes = [C.c_uint32] * 5 + [f32p]
+        L.oracle_draw_vertex.argtypes = [C.c_uint32] * 6 + [f32p]
+        L.oracle_env_cdf.argtypes = [f32p, C.c_int, C.c_int, f32p, f32p, f32p]
+        L.oracle_morton30.argtypes = [C.c_float, C.c_float, C.c_float, f32p, f32p]
+        L.oracle_morton30.restype = C.c_uint32
+        L.oracle_scene_create.restype = vp
+        L.oracle_scene_destroy.argtypes = [vp]
+        L.oracle_scene_set_geometry.argtypes = [vp, C.c_uint32, f32p, f32p, f32p, u32p]
+        L.oracle_scene_set_materials.argtypes = [vp, vp, C.c_uint32]
+        L.oracle_scene_add_texture.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, f32p]
+        L.oracle_scene_add_texture.restype = C.c_int
+        L.oracle_scene_set_texture_eval.argtypes = [vp, C.c_int, f32p]
+        L.oracle_scene_update_texture.argtypes = [vp, C.c_int, f32p]
+        L.oracle_scene_set_lights.argtypes = [vp, vp, C.c_uint32]
+        L.oracle_scene_set_env.argtypes = [vp, f32p, C.c_uint32, C.c_uint32, C.c_float]
+        L.oracle_scene_set_camera.argtypes = [vp, vp, C.c_uint32, C.c_uint32]
+        L.oracle_scene_set_settings.argtypes = [vp, vp]
+        L.oracle_scene_commit.argtypes = [vp]
+        L.oracle_scene_rebuild_emitters.argtypes = [vp]
+        L.oracle_scene_bvh.argtypes = [vp, vp, f32p, u32p, u64p]
+        L.oracle_scene_bvh.restype = C.c_uint32
+        L.oracle_trace_batch.argtypes = [vp, f32p, C.c_uint32, f32p, u32p, u32p]
+        L.oracle_occluded_batch.argtypes = [vp, f32p, C.c_uint32, u32p]
+        L.oracle_brute_batch.argtypes = [vp, f32p, C.c_uint32, C.c_double, f64p, u32p, u32p, u32p, C.c_uint32]
+        L.oracle_render.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_int, f64p, f64p,
+                                    C.c_uint32, C.c_uint32]
+        L.oracle_path.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, f64p]
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference's own sampler/env-CDF C++ (oracle/_ref), or None if absent."""
+    global _ref
+    if _ref is None and REF_PATH.exists():
+        R = C.CDLL(str(REF_PATH))
+        R.ref_sobol_sample1d.argtypes = [C.c_uint32, C.c_uint32]
+        R.ref_sobol_sample1d.restype = C.c_float
+        R.ref_sobol_dirs.argtypes = [C.POINTER(C.c_uint32)]
+        R.ref_owen.argtypes = [C.c_uint32, C.c_uint32]
+        R.ref_owen.restype = C.c_uint32
+        R.ref_env_cdf.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_float),
+                                  C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        _ref = R
+    return _ref
+
+
+def _p(a, t=C.c_float):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+class OracleScene:
+    """CPU scene mirroring ohao_engine_amd.scenes.Scene."""
+
+    def __init__(self, sc, settings_override=None):
+        self.L = lib()
+        self.h = self.L.oracle_scene_create()
+        self.sc = sc
+        self.pos = np.ascontiguousarray(sc.pos, np.float32)
+        self.nrm = None if sc.nrm is None else np.ascontiguousarray(sc.nrm, np.float32)
+        self.uv = None if sc.uv is None else np.ascontiguousarray(sc.uv, np.float32)
+        self.mat = np.ascontiguousarray(sc.mat, np.uint32)
+        self.L.oracle_scene_set_geometry(self.h, self.pos.shape[0], _p(self.pos), _p(self.nrm),
+                                         _p(self.uv), _p(self.mat, C.c_uint32))
+        self.set_materials(sc.materials)
+        for t in sc.textures:
+            t = np.ascontiguousarray(t, np.float32)
+            self.L.oracle_scene_add_texture(self.h, t.shape[1], t.shape[0], t.shape[2], _p(t))
+        lights = np.ascontiguousarray(sc.lights, np.float32).reshape(-1, 20)
+        self.L.oracle_scene_set_lights(self.h, lights.ctypes.data, lights.shape[0])
+        if sc.env is not None:
+            env = np.ascontiguousarray(sc.env, np.float32)
+            self.L.oracle_scene_set_env(self.h, _p(env), env.shape[1], env.shape[0], sc.env_intensity)
+        self.set_camera(sc.camera, sc.width, sc.height)
+        self.set_settings(settings_override or sc.settings)
+        self.L.oracle_scene_commit(self.h)
+
+    def set_materials(self, mats):
+        self._mats = np.ascontiguousarray(mats)
+        self.L.oracle_scene_set_materials(self.h, self._mats.ctypes.data, self._mats.shape[0])
+        self.L.oracle_scene_rebuild_emitters(self.h)
+
+    def set_camera(self, cam, W, H):
+        arr = np.concatenate([cam.position, cam.forward, cam.right, cam.up,
+                              [cam.tan_half_fov_y]]).astype(np.float32)
+        self._cam = arr
+        self.W, self.H = W, H
+        self.L.oracle_scene_set_camera(self.h, arr.ctypes.data, W, H)
+
+    def set_settings(self, s):
+        arr = np.zeros(8, np.uint32)
+        arr[0], arr[1], arr[2], arr[3] = s.max_bounces, s.rr_start, s.sampler, s.flags
+        arr[4] = np.float32(s.firefly_clamp).view(np.uint32)
+        arr[5] = np.float32(s.ray_eps).view(np.uint32)
+        self._st = arr
+        self.L.oracle_scene_set_settings(self.h, arr.ctypes.data)
+
+    def set_texture_eval(self, tid, data):
+        d = None if data is None else np.ascontiguousarray(data, np.float32)
+        self._te = d
+        self.L.oracle_scene_set_texture_eval(self.h, tid, _p(d))
+
+    def update_texture(self, tid, data):
+        d = np.ascontiguousarray(data, np.float32)
+        self.L.oracle_scene_update_texture(self.h, tid, _p(d))
+
+    def bvh(self):
+        n = self.pos.shape[0]
+        ni = max(n - 1, 1)
+        nodes = np.zeros((ni, 16), np.uint32)
+        tris = np.zeros((n, 12), np.float32)
+        prim = np.zeros(n, np.uint32)
+        keys = np.zeros(n, np.uint64)
+        self.L.oracle_scene_bvh(self.h, nodes.ctypes.data, _p(tris), _p(prim, C.c_uint32),
+                                _p(keys, C.c_uint64))
+        return nodes, tris, prim, keys
+
+    def trace(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.float32)
+        visits = np.zeros(n, np.uint32)
+        tests = np.zeros(n, np.uint32)
+        self.L.oracle_trace_batch(self.h, _p(rays), n, _p(hits), _p(visits, C.c_uint32), _p(tests, C.c_uint32))
+        return hits, visits, tests
+
+    def occluded(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(rays.shape[0], np.uint32)
+        self.L.oracle_occluded_batch(self.h, _p(rays), rays.shape[0], _p(out, C.c_uint32))
+        return out
+
+    def brute(self, rays, rel_tie=1e-5, max_ties=8):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        t = np.zeros(n, np.float64)
+        prim = np.zeros(n, np.uint32)
+        nt = np.zeros(n, np.uint32)
+        ties = np.zeros((n, max_ties), np.uint32)
+        self.L.oracle_brute_batch(self.h, _p(rays), n, rel_tie, _p(t, C.c_double), _p(prim, C.c_uint32),
+                                  _p(nt, C.c_uint32), _p(ties, C.c_uint32), max_ties)
+        return t, prim, nt, ties
+
+    def render(self, spp, first_sample, precision=64, threads=0, rows=None):
+        W, H = self.W, self.H
+        s = np.zeros((H, W, 3), np.float64)
+        q = np.zeros((H, W, 3), np.float64)
+        r0, r1 = rows if rows is not None else (0, H)
+        self.L.oracle_render(self.h, spp, first_sample, precision, threads, _p(s, C.c_double),
+                             _p(q, C.c_double), r0, r1)
+        return s, q
+
+    def path(self, px, py, sidx, precision=64):
+        out = np.zeros(3, np.float64)
+        self.L.oracle_path(self.h, px, py, sidx, precision, _p(out, C.c_double))
+        return out
+
+    def __del__(self):
+        try:
+            self.L.oracle_scene_destroy(self.h)
+        except Exception:
+            pass
+
+
+def env_cdf(env):
+    env = np.ascontiguousarray(env, np.float32)
+    H, W = env.shape[:2]
+    m = np.zeros(H, np.float32)
+    c = np.zeros(W * H, np.float32)
+    I = np.zeros(1, np.float32)
+    lib().oracle_env_cdf(_p(env), W, H, _p(m), _p(c), _p(I))
+    return m, c.reshape(H, W), float(I[0])
+
+
+def ref_env_cdf(env):
+    R = ref()
+    env = np.ascontiguousarray(env, np.float32)
+    H, W = env.shape[:2]
+    m = np.zeros(H, np.float32)
+    c = np.zeros(W * H, np.float32)
+    I = np.zeros(1, np.float32)
+    R.ref_env_cdf(_p(env), W, H, _p(m), _p(c), _p(I))
+    return m, c.reshape(H, W), float(I[0])
