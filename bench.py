@@ -48,19 +48,19 @@ def cpu_baseline(sc, seconds, threads):
     t0 = time.time()
     o = oracle_lib.OracleScene(sc)
     log(f"cpu baseline: oracle scene + LBVH in {time.time() - t0:.1f}s")
-    rows = 8
+    # whole 1920x1080 frames at 1 spp (consecutive sample indices) until ~`seconds` of CPU work
+    frames = 0
     t = time.time()
-    o.render(1, sc.settings.seed, precision=32, threads=threads, rows=(0, rows))
-    dt = max(time.time() - t, 1e-3)
-    rows = int(min(sc.height, max(rows, rows * seconds / dt)))
-    start = (sc.height - rows) // 2
-    t = time.time()
-    o.render(1, sc.settings.seed, precision=32, threads=threads, rows=(start, start + rows))
+    while True:
+        o.render(1, sc.settings.seed + frames, precision=32, threads=threads)
+        frames += 1
+        if time.time() - t >= seconds:
+            break
     dt = time.time() - t
-    paths = rows * sc.width
+    paths = frames * sc.width * sc.height
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"rows {start}..{start + rows - 1} of the 1920x1080 frame ({paths} paths, 1 spp, "
-                      f"depth 4), FP32 oracle restatement, {dt:.1f}s"}
+            "sample": f"{frames} full 1920x1080 frames at 1 spp, depth 4 ({paths} paths) of the bench workload, "
+                      f"FP32 oracle restatement (same LBVH, traversal and integrator), {dt:.1f}s"}
 
 
 def main():
@@ -129,6 +129,26 @@ def main():
     breakdown = {k: round(acc[k] / a.steps, 4) for k in ("ms_total", "ms_extend", "ms_shade", "ms_connect")}
     breakdown["extend_rays_per_step"] = acc["rays_extend"] // a.steps
     breakdown["shadow_rays_per_step"] = acc["rays_shadow"] // a.steps
+
+    # path-replay adjoint of the same frame: dL/dI = per-pixel weights, gradients
+    # scattered into the 1024x512 ground albedo texture
+    adj_steps = max(1, a.steps // 2)
+    w = np.random.default_rng(0).uniform(0.5, 1.5, (sc.height, sc.width, 3)).astype(np.float32)
+    ctx.zero_gradients()
+    ctx.adjoint(w, spp)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    adj_ms = 0.0
+    for i in range(adj_steps):
+        ctx.adjoint(w, spp)
+        adj_ms += ctx.stats().ms_total
+    adj_wall = time.perf_counter() - t0
+    adjoint = {"value": round(sc.width * sc.height * spp * adj_steps / adj_wall / 1e6, 3), "unit": "Mpaths/s",
+               "ms_per_step": round(adj_wall * 1e3 / adj_steps, 4),
+               "gpu_ms_per_step": round(adj_ms / adj_steps, 4),
+               "cost_vs_forward": round((adj_wall / adj_steps) / (wall / a.steps), 3),
+               "parameters": "1024x512 RGB ground albedo texture", "mode": "fp32 atomics"}
+    log(f"adjoint: {adjoint['ms_per_step']} ms/step, {adjoint['cost_vs_forward']}x forward")
     ctx.close()
 
     cpu = None
@@ -148,7 +168,7 @@ def main():
                                   "NEE+MIS, PCG32 seed 7", "triangles": sc.num_triangles, "width": sc.width,
                       "height": sc.height, "spp_per_step": spp, "max_bounces": sc.settings.max_bounces,
                       "lbvh_build_ms": round(build_ms, 3)},
-           "breakdown": breakdown, "roofline": roofline, "cpu_baseline": cpu}
+           "breakdown": breakdown, "adjoint": adjoint, "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 

@@ -225,6 +225,23 @@ int ohpt_read_pixels(ohpt_context* ctx, uint8_t* out, int on_device);
  * albedo RGBA32F, normal RGBA32F (world normal, w=1 on hit), depth F32. Any may be NULL. */
 int ohpt_read_aovs(ohpt_context* ctx, float* albedo, float* normal, float* depth, int on_device);
 
+/* ---- differentiation ---------------------------------------------------------
+ * Path-replay adjoint of the image formed by the first `spp` samples after a
+ * reset (sample indices seed .. seed+spp-1). dl_dimage is dL/d(mean radiance)
+ * per pixel, RGBA32F (alpha ignored). Gradients with respect to every texel of
+ * every texture bound as albedo (RGB), roughness or metallic (channel 0)
+ * accumulate into per-texture buffers. Replaces the coordinate finite
+ * differences of StagedFitter::runStage (ohao/inverse/staged_fit.hpp:296-357)
+ * and turns IInverseImageFormation::supportsAnalyticGrads
+ * (ohao/inverse/backend/image_formation.hpp:53) on for the path tracer.
+ * Requires max_bounces <= 15. With OHPT_ADJOINT_DETERMINISTIC the texel sums
+ * are order-independent (bit-reproducible); otherwise FP32 atomics. */
+enum { OHPT_ADJOINT_DETERMINISTIC = 1u << 0 };
+int ohpt_adjoint(ohpt_context* ctx, const float* dl_dimage, int on_device, uint32_t spp, uint32_t flags);
+int ohpt_zero_gradients(ohpt_context* ctx);
+/* dL/d(texel) of texture `id`, w*h*channels floats (the upload layout). */
+int ohpt_read_texture_gradient(ohpt_context* ctx, int32_t id, float* out, int on_device);
+
 /* Trace a batch of caller-supplied rays (closest hit). rays: n x 8 floats
  * (ox,oy,oz,tmin, dx,dy,dz,tmax); hits: n x 4 (t, prim as uint bits, u, v),
  * prim = 0xFFFFFFFF on miss. prim is the caller's global triangle numbering

@@ -63,6 +63,7 @@ struct DevBuf {
 struct TexStore {
     DevBuf data;
     uint32_t w = 0, h = 0, c = 0;
+    size_t packed_floats() const { return (size_t)w * h * (c >= 2 ? 4u : 1u); }
 };
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -111,6 +112,10 @@ struct ohpt_context {
     DevBuf ps_o, ps_d, ps_T, ps_L, ps_ln0, ps_ln1, ps_aov0, ps_aov1, ps_q0, ps_q1, ps_hits, sh_o, sh_d, sh_c,
         counters, ovf, tmp, tstats;
     uint32_t ovf_rays = 0;
+    DevBuf adj_vis, adj_lrem, adj_dldi;     // path-replay adjoint state
+    DevBuf grad_arena;                      // dL/d(texel) of all textures, packed like the textures
+    std::vector<size_t> grad_off;           // per-texture offset in the arena (floats)
+    DevBuf rec_key, rec_val, rec_scratch;   // deterministic-mode gradient records
 
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> ev_pool;
@@ -180,8 +185,8 @@ int ensure_paths(ohpt_context* c, uint32_t n) {
     HIPCHK(c->sh_o.ensure(2 * N * 16));
     HIPCHK(c->sh_d.ensure(2 * N * 16));
     HIPCHK(c->sh_c.ensure(2 * N * 16));
-    HIPCHK(c->counters.ensure(128 * 4));
-    HIPCHK(c->ovf.ensure(2 * N * kStackOvf * 4));
+    HIPCHK(c->counters.ensure(kCounterWords * 4));
+    HIPCHK(c->ovf.ensure(trace_overflow_bytes()));
     c->ovf_rays = 2 * n;
     c->capacity = n;
     return OHPT_OK;
@@ -283,10 +288,12 @@ int upload_texdesc(ohpt_context* c) {
     std::vector<TexDesc> td(std::max<size_t>(c->textures.size(), 1));
     for (size_t i = 0; i < c->textures.size(); ++i) {
         td[i].data = c->textures[i].data.as<float>();
+        const bool has_grad = c->grad_arena.p && i < c->grad_off.size();
+        td[i].grad = has_grad ? c->grad_arena.as<float>() + c->grad_off[i] : nullptr;
+        td[i].grad_base = has_grad ? (uint32_t)c->grad_off[i] : 0u;
         td[i].w = c->textures[i].w;
         td[i].h = c->textures[i].h;
         td[i].c = c->textures[i].c;
-        td[i].pad = 0;
     }
     HIPCHK(c->d_texdesc.ensure(td.size() * sizeof(TexDesc)));
     HIPCHK(hipMemcpyAsync(c->d_texdesc.p, td.data(), td.size() * sizeof(TexDesc), hipMemcpyHostToDevice,
@@ -732,7 +739,7 @@ int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
     uint32_t n_extend = 0;
     HIPCHK(hipEventRecord(c->ev_start, c->stream));
     uint64_t rays_ext = 0, rays_sh = 0;
-    std::vector<uint32_t> host_counters(128);
+    std::vector<uint32_t> host_counters(kCounterWords);
     uint32_t remaining = spp;
     while (remaining > 0) {
         const uint32_t batch = std::min(remaining, per_batch);
@@ -748,13 +755,13 @@ int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
         f.flags = c->st.flags;
         f.firefly_clamp = c->st.firefly_clamp;
         f.ray_eps = c->st.ray_eps;
-        launch_fill_u32(ps.counters, 0u, 128, c->stream);
+        launch_fill_u32(ps.counters, 0u, kCounterWords, c->stream);
         launch_generate(f, ps, c->stream);
         for (uint32_t b = 0; b <= f.max_bounces; ++b) {
             Seg sg{};
             if ((r = mark(&sg.a))) return r;
-            launch_extend(S, ps.q[b & 1u], ps.counters + b, f.n_paths, ps.o, ps.d, ps.hits, c->ovf.as<uint32_t>(),
-                          tstats, c->stream);
+            launch_extend(S, ps.q[b & 1u], ps.counters + b, f.n_paths, ps.o, ps.d, ps.hits,
+                          ps.counters + kExtendFetchBase + b, c->ovf.as<uint32_t>(), tstats, c->stream);
             ++n_extend;
             if ((r = mark(&sg.b))) return r;
             if (profile) { sg.kind = 0; segs.push_back(sg); }
@@ -765,7 +772,8 @@ int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
             if (b < f.max_bounces) {
                 if ((r = mark(&sg.a))) return r;
                 launch_connect(S, ps.counters + kShadowCounterBase + b, 2u * f.n_paths, ps.sh_o, ps.sh_d, ps.sh_c,
-                               ps.ln0, ps.ln1, c->ovf.as<uint32_t>(), c->stream);
+                               ps.ln0, ps.ln1, ps.counters + kConnectFetchBase + b, c->ovf.as<uint32_t>(),
+                               nullptr, b, c->stream);
                 if ((r = mark(&sg.b))) return r;
                 if (profile) { sg.kind = 2; segs.push_back(sg); }
             }
@@ -774,7 +782,7 @@ int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
                           c->d_aov_normal.as<float4>(), c->stream);
         HIPCHK(hipGetLastError());
         if (!(render_flags & OHPT_RENDER_ASYNC)) {
-            HIPCHK(hipMemcpyAsync(host_counters.data(), ps.counters, 128 * 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(host_counters.data(), ps.counters, kCounterWords * 4, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
             for (uint32_t b = 0; b <= f.max_bounces; ++b) rays_ext += host_counters[b];
             for (uint32_t b = 0; b < f.max_bounces; ++b) rays_sh += host_counters[kShadowCounterBase + b];
@@ -805,6 +813,155 @@ int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
             c->stats.tri_tests = h[1];
         }
     }
+    return OHPT_OK;
+}
+
+// ---------------------------------------------------------------- adjoint
+int ohpt_zero_gradients(ohpt_context* c) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_zero_gradients: null context");
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<size_t> off;
+    size_t total = 0;
+    for (const TexStore& t : c->textures) {
+        off.push_back(total);
+        total += (t.packed_floats() + 63) & ~size_t(63);
+    }
+    if (total >= 0xFFFFFFFFull) return fail(OHPT_ERR_UNSUPPORTED, "ohpt_zero_gradients: textures too large");
+    const bool changed = off != c->grad_off || !c->grad_arena.p;
+    if (changed) {
+        HIPCHK(c->grad_arena.ensure(std::max<size_t>(total, 64) * 4));
+        c->grad_off = off;
+    }
+    launch_fill_u32(c->grad_arena.as<uint32_t>(), 0u, (uint32_t)std::max<size_t>(total, 64), c->stream);
+    HIPCHK(hipGetLastError());
+    if (changed) return upload_texdesc(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+int ohpt_read_texture_gradient(ohpt_context* c, int32_t id, float* out, int on_device) {
+    if (!c || !out || id < 0 || id >= (int32_t)c->textures.size())
+        return fail(OHPT_ERR_INVALID_ARG, "ohpt_read_texture_gradient: bad arguments");
+    TexStore& t = c->textures[(size_t)id];
+    if (!c->grad_arena.p || (size_t)id >= c->grad_off.size())
+        return fail(OHPT_ERR_STATE, "ohpt_read_texture_gradient: no adjoint has run");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)t.w * t.h;
+    const uint32_t stride = t.c >= 2 ? 4u : 1u;
+    std::vector<float> packed(n * stride);
+    HIPCHK(hipMemcpyAsync(packed.data(), c->grad_arena.as<float>() + c->grad_off[(size_t)id], packed.size() * 4,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<float> dense(n * t.c);
+    for (size_t i = 0; i < n; ++i)
+        for (uint32_t k = 0; k < t.c; ++k) dense[i * t.c + k] = packed[i * stride + k];
+    if (on_device) HIPCHK(hipMemcpy(out, dense.data(), dense.size() * 4, hipMemcpyHostToDevice));
+    else std::memcpy(out, dense.data(), dense.size() * 4);
+    return OHPT_OK;
+}
+
+// Path-replay adjoint (Vicini, Speierer, Jakob 2021) of the image made of the
+// first `spp` samples after a reset (sample indices seed .. seed+spp-1):
+// pass 1 renders each batch forward and records shadow visibility per path;
+// pass 2 regenerates the same paths, re-traces them with the same random
+// numbers and scatters dL/d(texel) into the texture gradient buffers
+// (accumulating: call ohpt_zero_gradients first).
+int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_t spp, uint32_t flags) {
+    if (!c || !dl_dimage) return fail(OHPT_ERR_INVALID_ARG, "ohpt_adjoint: null argument");
+    if (!c->committed) return fail(OHPT_ERR_NO_SCENE, "ohpt_adjoint: no committed scene");
+    if (spp == 0) return OHPT_OK;
+    if (c->st.max_bounces > 15) return fail(OHPT_ERR_UNSUPPORTED, "ohpt_adjoint: max_bounces > 15");
+    if (flags & ~(uint32_t)OHPT_ADJOINT_DETERMINISTIC) return fail(OHPT_ERR_INVALID_ARG, "ohpt_adjoint: unknown flags");
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->grad_arena.p || c->grad_off.size() != c->textures.size()) {
+        int r = ohpt_zero_gradients(c);
+        if (r != OHPT_OK) return r;
+    }
+    const bool deterministic = (flags & OHPT_ADJOINT_DETERMINISTIC) != 0;
+    const uint64_t npix = (uint64_t)c->W * c->H;
+    HIPCHK(c->adj_dldi.ensure(npix * 16));
+    HIPCHK(hipMemcpyAsync(c->adj_dldi.p, dl_dimage, npix * 16,
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+    uint64_t target = std::max<uint64_t>(npix, 1ull << 22);
+    const uint64_t recs_per_path = (uint64_t)std::max(1u, c->st.max_bounces) * kGradRecordsPerVertex;
+    if (deterministic) target = std::max<uint64_t>(npix, (1ull << 28) / recs_per_path);
+    if (npix * recs_per_path > 0xFFFFFFFFull && deterministic)
+        return fail(OHPT_ERR_UNSUPPORTED, "ohpt_adjoint: image too large for the deterministic mode");
+    const uint32_t per_batch = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(spp, target / npix));
+    int r = ensure_paths(c, (uint32_t)(npix * per_batch));
+    if (r != OHPT_OK) return r;
+    HIPCHK(c->adj_vis.ensure((size_t)c->capacity * 4));
+    HIPCHK(c->adj_lrem.ensure((size_t)c->capacity * 16));
+    uint32_t* rec_key = nullptr;
+    uint32_t* rec_val = nullptr;
+    if (deterministic) {
+        const size_t nrec = (size_t)npix * per_batch * recs_per_path;
+        HIPCHK(c->rec_key.ensure(nrec * 4));
+        HIPCHK(c->rec_val.ensure(nrec * 4));
+        HIPCHK(c->rec_scratch.ensure(radix_sort_scratch_bytes((uint32_t)nrec)));
+        rec_key = c->rec_key.as<uint32_t>();
+        rec_val = c->rec_val.as<uint32_t>();
+    }
+    const DevScene S = scene_view(c);
+    const PathState ps = path_state(c);
+    uint32_t* vis = c->adj_vis.as<uint32_t>();
+    float4* lrem = c->adj_lrem.as<float4>();
+    const float inv_spp = 1.0f / (float)spp;
+    HIPCHK(hipEventRecord(c->ev_start, c->stream));
+    uint32_t remaining = spp, sample = c->seed;
+    while (remaining > 0) {
+        const uint32_t batch = std::min(remaining, per_batch);
+        FrameParams f{};
+        f.cam = c->cam;
+        f.width = c->W;
+        f.height = c->H;
+        f.n_paths = (uint32_t)(npix * batch);
+        f.sample_base = sample;
+        f.max_bounces = c->st.max_bounces;
+        f.rr_start = c->st.rr_start;
+        f.sampler = c->st.sampler;
+        f.flags = c->st.flags & ~(uint32_t)OHPT_FLAG_AOVS;
+        f.firefly_clamp = c->st.firefly_clamp;
+        f.ray_eps = c->st.ray_eps;
+        // pass 1: forward, recording shadow visibility
+        launch_fill_u32(ps.counters, 0u, kCounterWords, c->stream);
+        launch_fill_u32(vis, 0u, f.n_paths, c->stream);
+        launch_generate(f, ps, c->stream);
+        for (uint32_t b = 0; b <= f.max_bounces; ++b) {
+            launch_extend(S, ps.q[b & 1u], ps.counters + b, f.n_paths, ps.o, ps.d, ps.hits,
+                          ps.counters + kExtendFetchBase + b, c->ovf.as<uint32_t>(), nullptr, c->stream);
+            launch_shade(S, f, ps, b, c->stream);
+            if (b < f.max_bounces)
+                launch_connect(S, ps.counters + kShadowCounterBase + b, 2u * f.n_paths, ps.sh_o, ps.sh_d, ps.sh_c,
+                               ps.ln0, ps.ln1, ps.counters + kConnectFetchBase + b, c->ovf.as<uint32_t>(), vis, b,
+                               c->stream);
+        }
+        launch_adjoint_init(ps, f.n_paths, lrem, c->stream);
+        // pass 2: replay with the same random numbers, scatter gradients
+        const uint32_t nrec = (uint32_t)(f.n_paths * recs_per_path);
+        if (deterministic) launch_fill_u32(rec_key, kNoRecord, nrec, c->stream);
+        launch_fill_u32(ps.counters, 0u, kCounterWords, c->stream);
+        launch_generate(f, ps, c->stream);
+        for (uint32_t b = 0; b <= f.max_bounces; ++b) {
+            launch_extend(S, ps.q[b & 1u], ps.counters + b, f.n_paths, ps.o, ps.d, ps.hits,
+                          ps.counters + kExtendFetchBase + b, c->ovf.as<uint32_t>(), nullptr, c->stream);
+            launch_shade_adjoint(S, f, ps, b, vis, lrem, c->adj_dldi.as<float4>(), inv_spp, rec_key, rec_val,
+                                 c->stream);
+        }
+        if (deterministic) {
+            // stable sort of the fixed-slot records by texel, then in-order segment sums
+            hipError_t e = radix_sort_pairs(rec_key, rec_val, nrec, 32, c->rec_scratch.p, c->stream);
+            if (e != hipSuccess) return fail_hip("radix_sort_pairs", e);
+            launch_segment_reduce(rec_key, rec_val, nrec, c->grad_arena.as<float>(), c->stream);
+        }
+        HIPCHK(hipGetLastError());
+        sample += batch;
+        remaining -= batch;
+    }
+    HIPCHK(hipEventRecord(c->ev_stop, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev_stop));
+    c->stats.ms_total = elapsed(c->ev_start, c->ev_stop);
+    c->stats.paths = npix * spp;
     return OHPT_OK;
 }
 
@@ -902,8 +1059,9 @@ int ray_batch(ohpt_context* c, const float* rays, void* out, uint32_t n, int on_
     if (!c->committed) return fail(OHPT_ERR_NO_SCENE, "ray batch: no committed scene");
     if (n == 0) return OHPT_OK;
     HIPCHK(hipSetDevice(c->device));
-    DevBuf dr, dout, dovf;
-    HIPCHK(dovf.ensure((size_t)n * kStackOvf * 4));
+    DevBuf dr, dout, dovf, dfetch;
+    HIPCHK(dovf.ensure(trace_overflow_bytes()));
+    HIPCHK(dfetch.ensure(256));
     const float4* r = reinterpret_cast<const float4*>(rays);
     void* o = out;
     const size_t out_bytes = (size_t)n * (occl ? 4 : 16);
@@ -915,8 +1073,8 @@ int ray_batch(ohpt_context* c, const float* rays, void* out, uint32_t n, int on_
         o = dout.p;
     }
     const DevScene S = scene_view(c);
-    if (occl) launch_occluded_rays(S, r, reinterpret_cast<uint32_t*>(o), n, dovf.as<uint32_t>(), c->stream);
-    else launch_trace_rays(S, r, reinterpret_cast<float4*>(o), n, dovf.as<uint32_t>(), c->stream);
+    if (occl) launch_occluded_rays(S, r, reinterpret_cast<uint32_t*>(o), n, dfetch.as<uint32_t>(), dovf.as<uint32_t>(), c->stream);
+    else launch_trace_rays(S, r, reinterpret_cast<float4*>(o), n, dfetch.as<uint32_t>(), dovf.as<uint32_t>(), c->stream);
     HIPCHK(hipGetLastError());
     if (!on_device) HIPCHK(hipMemcpyAsync(out, dout.p, out_bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

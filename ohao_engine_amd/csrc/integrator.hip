@@ -11,6 +11,12 @@
 // ids, compacted with wave-aggregated atomics; shadow rays go to a separate
 // queue and their radiance lands in per-path light/env accumulators (one
 // writer per bounce), so the image is independent of queue order.
+//
+// Floating-point contraction is off in this file: the forward and the
+// adjoint (replay) instantiations of k_shade must compute bit-identical
+// throughputs so that replayed paths take the same decisions.
+#pragma clang fp contract(off)
+
 #include "ohpt_internal.h"
 
 namespace ohpt {
@@ -100,7 +106,24 @@ __device__ __forceinline__ uint32_t cdf_upper(const float* cdf, uint32_t n, floa
     return lo < n ? lo : n - 1;
 }
 
-__global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f, PathState ps, uint32_t bounce) {
+// Adjoint-pass inputs (path replay): per-path shadow visibility recorded by the
+// forward pass, remaining radiance, and dL/d(mean image) per pixel.
+struct AdjointParams {
+    const uint32_t* vis;       // bit 2b+k: shadow ray k (0 light, 1 env) of bounce b was unoccluded
+    float4* lrem;              // per-path radiance still to be explained by later vertices
+    const float4* dl_dimage;   // per-pixel dL/dI (RGB)
+    float inv_spp;             // 1 / samples per pixel of the image being differentiated
+    uint32_t* rec_key;         // deterministic mode: gradient records (null = atomics)
+    uint32_t* rec_val;
+};
+
+// One shading step of the wavefront integrator. kAdjoint = false: forward
+// (emits shadow rays and the next extend ray). kAdjoint = true: path replay of
+// the same step with the same random numbers and visibility, accumulating
+// dL/d(material textures).
+template <bool kAdjoint>
+__global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f, PathState ps, uint32_t bounce,
+                                                       AdjointParams adj) {
     const uint32_t* __restrict__ qin = ps.q[bounce & 1u];
     uint32_t* __restrict__ qout = ps.q[(bounce + 1u) & 1u];
     const uint32_t count = ps.counters[bounce];
@@ -118,8 +141,18 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
         const float prev_pdf = dd.w;
         float4 T4 = ps.T[pid];
         f3 T = xyz(T4);
-        float4 L4 = ps.L[pid];
+        float4 L4 = make_float4(0, 0, 0, 0);
+        if (!kAdjoint) L4 = ps.L[pid];
         f3 L = xyz(L4);
+        const uint32_t npix = f.width * f.height;
+        const uint32_t pixel = pid % npix;
+        f3 Lrem = mk(0, 0, 0), delta = mk(0, 0, 0);
+        uint32_t vis = 0;
+        if (kAdjoint) {
+            Lrem = xyz(adj.lrem[pid]);
+            delta = xyz(adj.dl_dimage[pixel]) * adj.inv_spp;
+            vis = adj.vis[pid];
+        }
         const bool light_nee = !(f.flags & OHPT_FLAG_NO_LIGHT_NEE);
         const bool env_nee = !(f.flags & OHPT_FLAG_NO_ENV_NEE) && S.env_w > 0;
         const uint32_t NA = S.n_lights;
@@ -130,8 +163,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
                 const f3 Le = env_eval(S, d);
                 const float w = (bounce == 0 || !env_nee) ? 1.0f : balance(prev_pdf, env_pdf(S, d));
                 L = L + T * Le * w;
+                if (kAdjoint) Lrem = Lrem - T * Le * w;
             }
-            if (bounce == 0 && (f.flags & OHPT_FLAG_AOVS)) {
+            if (!kAdjoint && bounce == 0 && (f.flags & OHPT_FLAG_AOVS)) {
                 ps.aov0[pid] = make_float4(0, 0, 0, 0);
                 ps.aov1[pid] = make_float4(0, 0, 0, 0);
             }
@@ -146,21 +180,23 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
                     w = balance(prev_pdf, pa * hit.x * hit.x / fmaxf(cl, 1e-30f));
                 }
                 L = L + T * Le * w;
+                if (kAdjoint) Lrem = Lrem - T * Le * w;
             }
             const Material m = load_material(S, s.mat, s.u, s.v);
-            if (bounce == 0 && (f.flags & OHPT_FLAG_AOVS)) {
+            if (!kAdjoint && bounce == 0 && (f.flags & OHPT_FLAG_AOVS)) {
                 ps.aov0[pid] = f4(m.albedo, hit.x);
                 ps.aov1[pid] = f4(s.ns, 1.0f);
             }
             if (bounce < f.max_bounces) {
-                const uint32_t npix = f.width * f.height;
-                const uint32_t pixel = pid % npix, ks = pid / npix;
+                const uint32_t ks = pid / npix;
                 float u[kDimsPerVertex];
                 draw_dims(f.sampler, pixel % f.width, pixel / f.width, f.width, f.sample_base + ks, 1u + bounce,
                           kDimsPerVertex, u);
                 const f3 wo = -d;
                 const float eps = offset_eps(f, s.p);
                 const f3 po = s.p + s.ng * eps;
+                f3 gA = mk(0, 0, 0);
+                float gR = 0.0f, gM = 0.0f;
                 // ---- NEE: one light
                 if (light_nee && NL > 0) {
                     uint32_t k = (uint32_t)(u[0] * (float)NL);
@@ -173,11 +209,19 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
                             if (ci > 0.0f && dot(s.ng, wi) > 0.0f) {
                                 const f3 fb = bsdf_eval(m, s.ns, wo, wi);
                                 if (maxc(fb) > 0.0f) {
-                                    const float tm = isinf(dist) ? INFINITY : dist * (1.0f - 1e-4f) - eps;
-                                    shl_o = f4(po, tm);
-                                    shl_d = f4(wi, __uint_as_float(pid * 2u));
-                                    shl_c = f4(T * fb * Lw * (ci * (float)NL), 0.0f);
-                                    emit_l = true;
+                                    const f3 K = T * Lw * (ci * (float)NL);
+                                    if (kAdjoint) {
+                                        if (vis & (1u << (2u * bounce))) {
+                                            Lrem = Lrem - fb * K;
+                                            bsdf_grad(m, s.ns, wo, wi, delta * K, gA, gR, gM);
+                                        }
+                                    } else {
+                                        const float tm = isinf(dist) ? INFINITY : dist * (1.0f - 1e-4f) - eps;
+                                        shl_o = f4(po, tm);
+                                        shl_d = f4(wi, __uint_as_float(pid * 2u));
+                                        shl_c = f4(fb * K, 0.0f);
+                                        emit_l = true;
+                                    }
                                 }
                             }
                         }
@@ -202,10 +246,18 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
                                 const f3 fb = bsdf_eval(m, s.ns, wo, wi);
                                 if (maxc(fb) > 0.0f && pl > 0.0f) {
                                     const float w = balance(pl, bsdf_pdf(m, s.ns, wo, wi));
-                                    shl_o = f4(po, dist * (1.0f - 1e-4f) - eps);
-                                    shl_d = f4(wi, __uint_as_float(pid * 2u));
-                                    shl_c = f4(T * fb * Le2 * (ci * w / pl), 0.0f);
-                                    emit_l = true;
+                                    const f3 K = T * Le2 * (ci * w / pl);
+                                    if (kAdjoint) {
+                                        if (vis & (1u << (2u * bounce))) {
+                                            Lrem = Lrem - fb * K;
+                                            bsdf_grad(m, s.ns, wo, wi, delta * K, gA, gR, gM);
+                                        }
+                                    } else {
+                                        shl_o = f4(po, dist * (1.0f - 1e-4f) - eps);
+                                        shl_d = f4(wi, __uint_as_float(pid * 2u));
+                                        shl_c = f4(fb * K, 0.0f);
+                                        emit_l = true;
+                                    }
                                 }
                             }
                         }
@@ -221,15 +273,23 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
                             const f3 fb = bsdf_eval(m, s.ns, wo, wi);
                             if (maxc(fb) > 0.0f) {
                                 const float w = balance(pe, bsdf_pdf(m, s.ns, wo, wi));
-                                she_o = f4(po, INFINITY);
-                                she_d = f4(wi, __uint_as_float(pid * 2u + 1u));
-                                she_c = f4(T * fb * env_eval(S, wi) * (ci * w / pe), 0.0f);
-                                emit_e = true;
+                                const f3 K = T * env_eval(S, wi) * (ci * w / pe);
+                                if (kAdjoint) {
+                                    if (vis & (2u << (2u * bounce))) {
+                                        Lrem = Lrem - fb * K;
+                                        bsdf_grad(m, s.ns, wo, wi, delta * K, gA, gR, gM);
+                                    }
+                                } else {
+                                    she_o = f4(po, INFINITY);
+                                    she_d = f4(wi, __uint_as_float(pid * 2u + 1u));
+                                    she_c = f4(fb * K, 0.0f);
+                                    emit_e = true;
+                                }
                             }
                         }
                     }
                 }
-                // ---- Russian roulette, then BSDF sampling
+                // ---- Russian roulette (probability detached), then BSDF sampling
                 bool alive = true;
                 if (bounce >= f.rr_start) {
                     const float qc = fminf(1.0f, maxc(T));
@@ -243,6 +303,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
                         const float pdf = bsdf_pdf(m, s.ns, wo, wi);
                         if (ci > 0.0f && dot(s.ng, wi) > 0.0f && pdf > 0.0f) {
                             const f3 fb = bsdf_eval(m, s.ns, wo, wi);
+                            if (kAdjoint) {
+                                // radiance of the rest of the path is linear in fb: d/dθ = Lrem (x) dfb/fb
+                                const f3 K = mk(fb.x > 0.0f ? Lrem.x / fb.x : 0.0f, fb.y > 0.0f ? Lrem.y / fb.y : 0.0f,
+                                                fb.z > 0.0f ? Lrem.z / fb.z : 0.0f);
+                                bsdf_grad(m, s.ns, wo, wi, delta * K, gA, gR, gM);
+                            }
                             T = T * fb * (ci / pdf);
                             if (maxc(T) > 0.0f) {
                                 next_o = f4(po, 0.0f);
@@ -252,9 +318,18 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
                         }
                     }
                 }
+                if (kAdjoint) {
+                    GradSink sink{adj.rec_key, adj.rec_val,
+                                  (pid * f.max_bounces + bounce) * kGradRecordsPerVertex, 0u};
+                    scatter_material_grad(S, m, s.mat, s.u, s.v, gA, gR, gM, sink);
+                }
             }
         }
-        ps.L[pid] = f4(L, L4.w);
+        if (kAdjoint) {
+            adj.lrem[pid] = f4(Lrem, 0.0f);
+        } else {
+            ps.L[pid] = f4(L, L4.w);
+        }
         if (emit_next) {
             ps.T[pid] = f4(T, T4.w);
             ps.o[pid] = next_o;
@@ -262,13 +337,40 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
         }
     }
     // compaction (all lanes participate in the ballots)
-    uint32_t* cnt_sh = ps.counters + kShadowCounterBase + bounce;
-    const uint32_t sl = wave_append(cnt_sh, emit_l);
-    const uint32_t se = wave_append(cnt_sh, emit_e);
+    if (!kAdjoint) {
+        uint32_t* cnt_sh = ps.counters + kShadowCounterBase + bounce;
+        const uint32_t sl = wave_append(cnt_sh, emit_l);
+        const uint32_t se = wave_append(cnt_sh, emit_e);
+        if (emit_l) { ps.sh_o[sl] = shl_o; ps.sh_d[sl] = shl_d; ps.sh_c[sl] = shl_c; }
+        if (emit_e) { ps.sh_o[se] = she_o; ps.sh_d[se] = she_d; ps.sh_c[se] = she_c; }
+    }
     const uint32_t sn = wave_append(ps.counters + bounce + 1, emit_next);
-    if (emit_l) { ps.sh_o[sl] = shl_o; ps.sh_d[sl] = shl_d; ps.sh_c[sl] = shl_c; }
-    if (emit_e) { ps.sh_o[se] = she_o; ps.sh_d[se] = she_d; ps.sh_c[se] = she_c; }
     if (emit_next) qout[sn] = pid;
+}
+
+// Deterministic gradient reduction: records sorted by key (stable, so equal
+// keys keep their fixed slot order); the first record of each run sums the run
+// in that order and adds it to the arena (one writer per key).
+__global__ __launch_bounds__(256) void k_segment_reduce(const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ vals, uint32_t n,
+                                                        float* __restrict__ arena) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (k == kNoRecord || (i > 0 && keys[i - 1] == k)) return;
+    float sum = 0.0f;
+    for (uint32_t t = i; t < n && keys[t] == k; ++t) sum += __uint_as_float(vals[t]);
+    arena[k] += sum;
+}
+
+// Remaining radiance for replay = the path's total radiance from the forward pass.
+__global__ __launch_bounds__(256) void k_adjoint_init(uint32_t n, const float4* __restrict__ L,
+                                                      const float4* __restrict__ ln0, const float4* __restrict__ ln1,
+                                                      float4* __restrict__ lrem) {
+    const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
+    if (pid >= n) return;
+    const float4 a = L[pid], b = ln0[pid], c = ln1[pid];
+    lrem[pid] = make_float4(a.x + b.x + c.x, a.y + b.y + c.y, a.z + b.z + c.z, 0.0f);
 }
 
 __global__ __launch_bounds__(256) void k_accumulate(FrameParams f, const float4* __restrict__ L,
@@ -372,7 +474,22 @@ void launch_generate(const FrameParams& f, const PathState& ps, hipStream_t s) {
 }
 
 void launch_shade(const DevScene& S, const FrameParams& f, const PathState& ps, uint32_t bounce, hipStream_t s) {
-    k_shade<<<div_up(f.n_paths, kShadeBlock), kShadeBlock, 0, s>>>(S, f, ps, bounce);
+    k_shade<false><<<div_up(f.n_paths, kShadeBlock), kShadeBlock, 0, s>>>(S, f, ps, bounce, AdjointParams{});
+}
+
+void launch_shade_adjoint(const DevScene& S, const FrameParams& f, const PathState& ps, uint32_t bounce,
+                          const uint32_t* vis, float4* lrem, const float4* dl_dimage, float inv_spp,
+                          uint32_t* rec_key, uint32_t* rec_val, hipStream_t s) {
+    const AdjointParams adj{vis, lrem, dl_dimage, inv_spp, rec_key, rec_val};
+    k_shade<true><<<div_up(f.n_paths, kShadeBlock), kShadeBlock, 0, s>>>(S, f, ps, bounce, adj);
+}
+
+void launch_segment_reduce(const uint32_t* keys, const uint32_t* vals, uint32_t n, float* arena, hipStream_t s) {
+    if (n) k_segment_reduce<<<div_up(n, 256), 256, 0, s>>>(keys, vals, n, arena);
+}
+
+void launch_adjoint_init(const PathState& ps, uint32_t n, float4* lrem, hipStream_t s) {
+    k_adjoint_init<<<div_up(n, 256), 256, 0, s>>>(n, ps.L, ps.ln0, ps.ln1, lrem);
 }
 
 void launch_accumulate(const FrameParams& f, const PathState& ps, float4* accum, float4* moments,

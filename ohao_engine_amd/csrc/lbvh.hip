@@ -125,39 +125,17 @@ __global__ __launch_bounds__(kRadixBlock) void k_radix_hist(const uint32_t* __re
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// exclusive scan of m entries by one block of 1024 threads
-__global__ __launch_bounds__(1024) void k_scan_exclusive(uint32_t* __restrict__ data, uint32_t m) {
-    __shared__ uint32_t part[1024];
-    const uint32_t per = (m + 1023) / 1024;
-    const uint32_t b = threadIdx.x * per, e = min(b + per, m);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += data[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;
-    for (uint32_t i = b; i < e; ++i) {
-        const uint32_t v = data[i];
-        data[i] = run;
-        run += v;
-    }
-}
-
 __global__ __launch_bounds__(kRadixBlock) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                const uint32_t* __restrict__ vin,
                                                                uint32_t* __restrict__ kout,
                                                                uint32_t* __restrict__ vout,
-                                                               const uint32_t* __restrict__ offsets, uint32_t n,
+                                                               const uint32_t* __restrict__ offsets,
+                                                               const uint32_t* __restrict__ digit_base, uint32_t n,
                                                                uint32_t shift, uint32_t nblocks) {
     __shared__ uint32_t s_base[kRadixDigits];
     __shared__ uint32_t s_wave[kRadixBlock / 64][kRadixDigits];
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63u;
-    s_base[t] = offsets[t * nblocks + blockIdx.x];
+    s_base[t] = offsets[t * nblocks + blockIdx.x] + digit_base[t];
     const uint32_t base = blockIdx.x * kRadixTile;
     for (uint32_t r = 0; r < kRadixItems; ++r) {
         for (uint32_t w = 0; w < kRadixBlock / 64; ++w) s_wave[w][t] = 0u;
@@ -281,16 +259,49 @@ __global__ void k_single_leaf(uint32_t* nodes_u, uint32_t* node_parent, uint32_t
     leaf_parent[0] = 0u;
 }
 
-__device__ __forceinline__ float ld_coherent(const float* p) {
-    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ __forceinline__ void st_coherent(float* p, float v) {
-    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+// --- level-synchronous refit -----------------------------------------------
+// Cross-CU communication inside one kernel would need device-scope release/
+// acquire (L2 write-back/invalidate on every XCD); instead nodes are bucketed
+// by depth and refitted one level per launch, deepest first, so every child
+// box was written by an earlier launch.
+constexpr uint32_t kMaxLevels = 64;
+
+__global__ __launch_bounds__(256) void k_node_depth(const uint32_t* __restrict__ node_parent, uint32_t ni,
+                                                    uint32_t* __restrict__ depth, uint32_t* __restrict__ level_count) {
+    __shared__ uint32_t h[kMaxLevels];
+    if (threadIdx.x < kMaxLevels) h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < ni) {
+        uint32_t d = 0, p = i;
+        while (node_parent[p] != kMiss && d < kMaxLevels - 1) { p = node_parent[p]; ++d; }
+        depth[i] = d;
+        atomicAdd(&h[d], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kMaxLevels && h[threadIdx.x]) atomicAdd(&level_count[threadIdx.x], h[threadIdx.x]);
 }
 
-__device__ __forceinline__ void child_box(uint32_t c, const float4* __restrict__ tris, const float* nbox,
+// level_cursor[d] starts at the level's offset; nodes are appended per level
+__global__ __launch_bounds__(256) void k_level_scatter(const uint32_t* __restrict__ depth, uint32_t ni,
+                                                       uint32_t* __restrict__ level_cursor,
+                                                       uint32_t* __restrict__ level_nodes) {
+    __shared__ uint32_t cnt[kMaxLevels], base[kMaxLevels];
+    if (threadIdx.x < kMaxLevels) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    uint32_t d = 0, r = 0;
+    if (i < ni) {
+        d = depth[i];
+        r = atomicAdd(&cnt[d], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kMaxLevels && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&level_cursor[threadIdx.x], cnt[threadIdx.x]);
+    __syncthreads();
+    if (i < ni) level_nodes[base[d] + r] = i;
+}
+
+__device__ __forceinline__ void child_box(uint32_t c, const float4* __restrict__ tris, const float* __restrict__ nodes_f,
                                           float lo[3], float hi[3]) {
     if (c & kLeafBit) {
         const size_t s = c & ~kLeafBit;
@@ -298,56 +309,35 @@ __device__ __forceinline__ void child_box(uint32_t c, const float4* __restrict__
         const float v[9] = {a.x, a.y, a.z, b.x, b.y, b.z, e.x, e.y, e.z};
         tri_bounds(v, lo, hi);
     } else {
+        const float* cb = nodes_f + (size_t)c * 16;
         for (int k = 0; k < 3; ++k) {
-            lo[k] = ld_coherent(&nbox[(size_t)c * 6 + k]);
-            hi[k] = ld_coherent(&nbox[(size_t)c * 6 + 3 + k]);
+            lo[k] = fminf(cb[k], cb[6 + k]);
+            hi[k] = fmaxf(cb[3 + k], cb[9 + k]);
         }
     }
 }
 
-// One thread per leaf climbs to the root; the second arrival at a node builds it.
-__global__ __launch_bounds__(256) void k_refit(const float4* __restrict__ tris, uint32_t n,
-                                               float* __restrict__ nodes_f, const uint32_t* __restrict__ node_parent,
-                                               const uint32_t* __restrict__ leaf_parent, uint32_t* __restrict__ flags,
-                                               float* __restrict__ nbox) {
-    const uint32_t s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= n) return;
-    uint32_t node = leaf_parent[s];
+__global__ __launch_bounds__(256) void k_refit_level(const float4* __restrict__ tris, float* __restrict__ nodes_f,
+                                                     const uint32_t* __restrict__ node_parent,
+                                                     const uint32_t* __restrict__ level_nodes, uint32_t count) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t node = level_nodes[t];
     const uint32_t* nodes_u = reinterpret_cast<const uint32_t*>(nodes_f);
-    while (node != kMiss) {
-        const uint32_t prev = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t need = (n == 1) ? 0u : 1u;  // a single-leaf root has one arrival
-        if (prev < need) return;
-        const uint32_t c0 = nodes_u[(size_t)node * 16 + 12], c1 = nodes_u[(size_t)node * 16 + 13];
-        float lo0[3], hi0[3], lo1[3], hi1[3];
-        child_box(c0, tris, nbox, lo0, hi0);
-        child_box(c1, tris, nbox, lo1, hi1);
-        float* nd = nodes_f + (size_t)node * 16;
-        for (int k = 0; k < 3; ++k) {
-            nd[k] = lo0[k]; nd[3 + k] = hi0[k]; nd[6 + k] = lo1[k]; nd[9 + k] = hi1[k];
-            st_coherent(&nbox[(size_t)node * 6 + k], fminf(lo0[k], lo1[k]));
-            st_coherent(&nbox[(size_t)node * 6 + 3 + k], fmaxf(hi0[k], hi1[k]));
-        }
-        reinterpret_cast<uint32_t*>(nd)[14] = node_parent[node];
-        __threadfence();
-        node = node_parent[node];
-    }
-}
-
-__global__ __launch_bounds__(256) void k_depth(const uint32_t* __restrict__ node_parent,
-                                               const uint32_t* __restrict__ leaf_parent, uint32_t n,
-                                               uint32_t* __restrict__ out) {
-    const uint32_t s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= n) return;
-    uint32_t d = 1, node = leaf_parent[s];
-    while (node != kMiss && d < 4096) { node = node_parent[node]; ++d; }
-    atomicMax(out, d);
+    const uint32_t c0 = nodes_u[(size_t)node * 16 + 12], c1 = nodes_u[(size_t)node * 16 + 13];
+    float lo0[3], hi0[3], lo1[3], hi1[3];
+    child_box(c0, tris, nodes_f, lo0, hi0);
+    child_box(c1, tris, nodes_f, lo1, hi1);
+    float* nd = nodes_f + (size_t)node * 16;
+    for (int k = 0; k < 3; ++k) { nd[k] = lo0[k]; nd[3 + k] = hi0[k]; nd[6 + k] = lo1[k]; nd[9 + k] = hi1[k]; }
+    reinterpret_cast<uint32_t*>(nd)[14] = node_parent[node];
 }
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 struct ScratchLayout {
-    size_t keys0, vals0, keys1, vals1, hist, partial, node_parent, leaf_parent, flags, nbox, depth, total;
+    size_t keys0, vals0, keys1, vals1, hist, digit, partial, node_parent, leaf_parent, node_depth, levels,
+        level_nodes, total;
 };
 ScratchLayout layout(uint32_t n) {
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -359,12 +349,13 @@ ScratchLayout layout(uint32_t n) {
     L.keys1 = o; o += al((size_t)n * 4);
     L.vals1 = o; o += al((size_t)n * 4);
     L.hist = o; o += al((size_t)kRadixDigits * nb * 4 + 4);
+    L.digit = o; o += al((size_t)kRadixDigits * 4 * 2);
     L.partial = o; o += al((size_t)(1024 + 1) * 6 * 4);
     L.node_parent = o; o += al((size_t)n * 4 + 4);
     L.leaf_parent = o; o += al((size_t)n * 4 + 4);
-    L.flags = o; o += al((size_t)n * 4 + 4);
-    L.nbox = o; o += al((size_t)n * 24 + 24);
-    L.depth = o; o += al(4);
+    L.node_depth = o; o += al((size_t)n * 4 + 4);
+    L.levels = o; o += al((size_t)kMaxLevels * 4 * 2);
+    L.level_nodes = o; o += al((size_t)n * 4 + 4);
     L.total = o;
     return L;
 }
@@ -374,11 +365,61 @@ __global__ void k_fill(uint32_t* p, uint32_t v, uint32_t n) {
     if (i < n) p[i] = v;
 }
 
+// Exclusive scan of one digit's per-block counts (one block per digit);
+// writes the digit's total.
+__global__ __launch_bounds__(256) void k_scan_digit(uint32_t* __restrict__ hist, uint32_t nb,
+                                                    uint32_t* __restrict__ totals) {
+    __shared__ uint32_t part[256];
+    uint32_t* row = hist + (size_t)blockIdx.x * nb;
+    const uint32_t per = (nb + 255) / 256;
+    const uint32_t b0 = threadIdx.x * per, e = min(b0 + per, nb);
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < e; ++i) s += row[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (uint32_t i = b0; i < e; ++i) {
+        const uint32_t v = row[i];
+        row[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 255) totals[blockIdx.x] = part[255];
+}
+
+// Exclusive scan of the 256 digit totals -> digit base offsets.
+__global__ __launch_bounds__(256) void k_scan_totals(const uint32_t* __restrict__ totals, uint32_t* __restrict__ base) {
+    __shared__ uint32_t part[256];
+    const uint32_t v0 = totals[threadIdx.x];
+    part[threadIdx.x] = v0;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    base[threadIdx.x] = part[threadIdx.x] - v0;
+}
+
+void radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t shift,
+                uint32_t nb, uint32_t* hist, uint32_t* digit, hipStream_t s) {
+    k_radix_hist<<<nb, kRadixBlock, 0, s>>>(kin, n, shift, nb, hist);
+    k_scan_digit<<<kRadixDigits, 256, 0, s>>>(hist, nb, digit);
+    k_scan_totals<<<1, 256, 0, s>>>(digit, digit + kRadixDigits);
+    k_radix_scatter<<<nb, kRadixBlock, 0, s>>>(kin, vin, kout, vout, hist, digit + kRadixDigits, n, shift, nb);
+}
+
 }  // namespace
 
 size_t radix_sort_scratch_bytes(uint32_t n) {
     const uint32_t nb = div_up(n, kRadixTile);
-    return ((size_t)n * 8 + 256) + ((size_t)kRadixDigits * nb * 4 + 256);
+    return ((size_t)n * 8 + 256) + ((size_t)kRadixDigits * nb * 4 + 256) + kRadixDigits * 8;
 }
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, void* scratch,
@@ -387,14 +428,14 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t
     const uint32_t nb = div_up(n, kRadixTile);
     uint32_t* k1 = reinterpret_cast<uint32_t*>(scratch);
     uint32_t* v1 = k1 + n;
-    uint32_t* hist = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(scratch) + (((size_t)n * 8 + 255) & ~size_t(255)));
+    const size_t hist_off = ((size_t)n * 8 + 255) & ~size_t(255);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(scratch) + hist_off);
+    uint32_t* digit = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(scratch) + hist_off +
+                                                  (((size_t)kRadixDigits * nb * 4 + 255) & ~size_t(255)));
     uint32_t *ka = keys, *va = vals, *kb = k1, *vb = v1;
     const uint32_t passes = (key_bits + 7) / 8;
     for (uint32_t p = 0; p < passes; ++p) {
-        const uint32_t shift = p * 8;
-        k_radix_hist<<<nb, kRadixBlock, 0, s>>>(ka, n, shift, nb, hist);
-        k_scan_exclusive<<<1, 1024, 0, s>>>(hist, kRadixDigits * nb);
-        k_radix_scatter<<<nb, kRadixBlock, 0, s>>>(ka, va, kb, vb, hist, n, shift, nb);
+        radix_pass(ka, va, kb, vb, n, p * 8, nb, hist, digit, s);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
@@ -418,11 +459,13 @@ hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out) 
     uint32_t* vals1 = reinterpret_cast<uint32_t*>(sc + L.vals1);
     uint32_t* hist = reinterpret_cast<uint32_t*>(sc + L.hist);
     float* partial = reinterpret_cast<float*>(sc + L.partial);
+    uint32_t* digit = reinterpret_cast<uint32_t*>(sc + L.digit);
     uint32_t* node_parent = reinterpret_cast<uint32_t*>(sc + L.node_parent);
     uint32_t* leaf_parent = reinterpret_cast<uint32_t*>(sc + L.leaf_parent);
-    uint32_t* flags = reinterpret_cast<uint32_t*>(sc + L.flags);
-    float* nbox = reinterpret_cast<float*>(sc + L.nbox);
-    uint32_t* depth = reinterpret_cast<uint32_t*>(sc + L.depth);
+    uint32_t* node_depth = reinterpret_cast<uint32_t*>(sc + L.node_depth);
+    uint32_t* level_count = reinterpret_cast<uint32_t*>(sc + L.levels);
+    uint32_t* level_cursor = level_count + kMaxLevels;
+    uint32_t* level_nodes = reinterpret_cast<uint32_t*>(sc + L.level_nodes);
 
     const uint32_t nparts = std::min<uint32_t>(1024u, div_up(n, 256));
     k_centroid_bounds<<<nparts, 256, 0, s>>>(b.tri_pos, n, partial);
@@ -434,10 +477,7 @@ hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out) 
     const uint32_t nb = div_up(n, kRadixTile);
     uint32_t *ka = keys0, *va = vals0, *kb = keys1, *vb = vals1;
     for (uint32_t p = 0; p < 4; ++p) {
-        const uint32_t shift = p * 8;
-        k_radix_hist<<<nb, kRadixBlock, 0, s>>>(ka, n, shift, nb, hist);
-        k_scan_exclusive<<<1, 1024, 0, s>>>(hist, kRadixDigits * nb);
-        k_radix_scatter<<<nb, kRadixBlock, 0, s>>>(ka, va, kb, vb, hist, n, shift, nb);
+        radix_pass(ka, va, kb, vb, n, p * 8, nb, hist, digit, s);
         std::swap(ka, kb);
         std::swap(va, vb);
     }
@@ -445,24 +485,37 @@ hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out) 
                                              b.prim, b.slot_of);
     uint32_t* nodes_u = reinterpret_cast<uint32_t*>(b.nodes);
     const uint32_t ni = n > 1 ? n - 1 : 1;
-    k_fill<<<div_up(ni, 256), 256, 0, s>>>(flags, 0u, ni);
-    k_fill<<<1, 256, 0, s>>>(depth, 0u, 1);
     if (n == 1) {
         k_single_leaf<<<1, 1, 0, s>>>(nodes_u, node_parent, leaf_parent);
     } else {
         k_karras<<<div_up(n - 1, 256), 256, 0, s>>>(ka, va, n, nodes_u, node_parent, leaf_parent);
     }
-    k_refit<<<div_up(n, 256), 256, 0, s>>>(b.tris, n, reinterpret_cast<float*>(b.nodes), node_parent, leaf_parent,
-                                           flags, nbox);
-    k_depth<<<div_up(n, 256), 256, 0, s>>>(node_parent, leaf_parent, n, depth);
-    hipError_t e = hipGetLastError();
+    // bucket internal nodes by depth, then refit level by level, deepest first
+    k_fill<<<1, 256, 0, s>>>(level_count, 0u, 2 * kMaxLevels);
+    k_node_depth<<<div_up(ni, 256), 256, 0, s>>>(node_parent, ni, node_depth, level_count);
+    uint32_t counts[kMaxLevels];
+    hipError_t e = hipMemcpyAsync(counts, level_count, sizeof(counts), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
-    if (depth_out) {
-        e = hipMemcpyAsync(depth_out, depth, 4, hipMemcpyDeviceToHost, s);
-        if (e != hipSuccess) return e;
-        e = hipStreamSynchronize(s);
+    uint32_t offs[kMaxLevels], maxd = 0, acc = 0;
+    for (uint32_t d = 0; d < kMaxLevels; ++d) {
+        offs[d] = acc;
+        acc += counts[d];
+        if (counts[d]) maxd = d;
     }
-    return e;
+    e = hipMemcpyAsync(level_cursor, offs, sizeof(offs), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    k_level_scatter<<<div_up(ni, 256), 256, 0, s>>>(node_depth, ni, level_cursor, level_nodes);
+    for (int d = (int)maxd; d >= 0; --d) {
+        if (counts[d] == 0) continue;
+        k_refit_level<<<div_up(counts[d], 256), 256, 0, s>>>(b.tris, reinterpret_cast<float*>(b.nodes), node_parent,
+                                                              level_nodes + offs[d], counts[d]);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // deepest leaf = two levels below the deepest internal node (root at 0, leaves counted)
+    if (depth_out) *depth_out = maxd + 2;
+    return hipStreamSynchronize(s);
 }
 
 }  // namespace ohpt

@@ -140,8 +140,14 @@ OHPT_DEV void draw_dims(uint32_t sampler, uint32_t px, uint32_t py, uint32_t wid
 // ------------------------------------------------------------------ scene view
 struct TexDesc {
     const float* data;     // float4 texels when c >= 2, float when c == 1
-    uint32_t w, h, c, pad;
+    float* grad;           // dL/d(texel), same layout, or null when not differentiated
+    uint32_t w, h, c;
+    uint32_t grad_base;    // offset of `grad` in the gradient arena (floats)
 };
+
+// Deterministic adjoint: per-vertex gradient records in fixed slots.
+constexpr uint32_t kGradRecordsPerVertex = 20;   // 4 texels x (3 albedo + roughness + metallic)
+constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 
 struct DevScene {
     const float4* nodes;
@@ -169,6 +175,7 @@ struct DevScene {
 struct Material {
     f3 albedo;
     float rough, metal, f0d, alpha;
+    float rough_raw, metal_raw;    // before clamping (gradients vanish outside the clamp range)
 };
 
 // ------------------------------------------------------------------ textures
@@ -220,6 +227,8 @@ OHPT_DEV Material load_material(const DevScene& S, uint32_t m, float u, float v)
     if (at >= 0) { const float4 t = tex_bilinear(S.tex[at], u, v); r.albedo = r.albedo * mk(t.x, t.y, t.z); }
     if (rt >= 0) r.rough *= tex_bilinear(S.tex[rt], u, v).x;
     if (mt >= 0) r.metal *= tex_bilinear(S.tex[mt], u, v).x;
+    r.rough_raw = r.rough;
+    r.metal_raw = r.metal;
     r.rough = fminf(1.0f, fmaxf(0.04f, r.rough));
     r.metal = fminf(1.0f, fmaxf(0.0f, r.metal));
     r.alpha = r.rough * r.rough;
@@ -416,6 +425,108 @@ OHPT_DEV bool bsdf_sample(const Material& m, f3 ns, f3 wo, float ul, float u1, f
 }
 
 OHPT_DEV float balance(float a, float b) { return a / fmaxf(a + b, 1e-30f); }
+
+// Adjoint of bsdf_eval: accumulates sum_c dK_c * d f_c / d(albedo_c, roughness,
+// metallic), where dK = adjoint weight (x) the factor multiplying f in the
+// contribution. Sampling pdfs, lobe probabilities and MIS weights are treated
+// as constants (detached), as path replay requires.
+OHPT_DEV void bsdf_grad(const Material& m, f3 ns, f3 wo, f3 wi, f3 dK, f3& gA, float& gR, float& gM) {
+    const float co = dot(ns, wo), ci = dot(ns, wi);
+    if (!(co > 0.0f) || !(ci > 0.0f)) return;
+    const f3 h = normalize(wo + wi);
+    const float ch = fmaxf(0.0f, dot(ns, h));
+    const float voh = fminf(1.0f, fmaxf(0.0f, dot(wo, h)));
+    const float a = m.alpha, a2 = a * a;
+    const float den = ch * ch * (a2 - 1.0f) + 1.0f;
+    const float D = a2 / (kPi * den * den);
+    const float dD = (2.0f * a / (kPi * den * den)) * (1.0f - 2.0f * a2 * ch * ch / den);
+    const float t2o = fmaxf(0.0f, 1.0f - co * co) / (co * co), t2i = fmaxf(0.0f, 1.0f - ci * ci) / (ci * ci);
+    const float ro = sqrtf(1.0f + a2 * t2o), ri = sqrtf(1.0f + a2 * t2i);
+    const float Lo = 0.5f * (-1.0f + ro), Li = 0.5f * (-1.0f + ri);
+    const float dLo = 0.5f * a * t2o / ro, dLi = 0.5f * a * t2i / ri;
+    const float G2 = 1.0f / (1.0f + Lo + Li);
+    const float dG2 = -G2 * G2 * (dLo + dLi);
+    const float q = 1.0f - voh;
+    const float s = q * q * q * q * q;
+    const float inv4 = 1.0f / (4.0f * co * ci);
+    const float sp = D * G2 * inv4;
+    const float dsp_da = (dD * G2 + D * dG2) * inv4;
+    const float kd = (1.0f - m.metal) * (1.0f / kPi);
+    const float F90 = (m.f0d > 0.0f || m.metal > 0.0f) ? 1.0f : 0.0f;
+    const float alb[3] = {m.albedo.x, m.albedo.y, m.albedo.z};
+    const float dk[3] = {dK.x, dK.y, dK.z};
+    float ga[3];
+    for (int c = 0; c < 3; ++c) {
+        const float F0 = m.f0d + (alb[c] - m.f0d) * m.metal;
+        const float F = F0 + (F90 - F0) * s;
+        const float dF_da = (1.0f - s) * m.metal;
+        const float dF_dm = (alb[c] - m.f0d) * (1.0f - s);
+        const float df_da = dF_da * (sp - kd * alb[c]) + (1.0f - F) * kd;
+        const float df_dm = dF_dm * (sp - kd * alb[c]) - (1.0f - F) * alb[c] * (1.0f / kPi);
+        const float df_dalpha = F * dsp_da;
+        ga[c] = dk[c] * df_da;
+        gM += dk[c] * df_dm;
+        gR += dk[c] * df_dalpha * 2.0f * m.rough;
+    }
+    gA = gA + mk(ga[0], ga[1], ga[2]);
+}
+
+// Gradient sink: FP32 atomics into the arena, or (deterministic mode) one
+// record per value in this vertex's fixed slots, reduced later in a fixed order.
+struct GradSink {
+    uint32_t* rec_key;     // null -> atomics
+    uint32_t* rec_val;
+    uint32_t rec_base;
+    uint32_t n;
+    OHPT_DEV void add(const TexDesc& t, size_t elem, float v) {
+        if (rec_key) {
+            if (n < kGradRecordsPerVertex) {
+                rec_key[rec_base + n] = t.grad_base + (uint32_t)elem;
+                rec_val[rec_base + n] = __float_as_uint(v);
+                ++n;
+            }
+        } else if (v != 0.0f) {
+            atomicAdd(t.grad + elem, v);
+        }
+    }
+};
+
+// Scatters material-parameter gradients at a surface point into the texel
+// gradients of the albedo/roughness/metallic textures (bilinear footprint).
+OHPT_DEV void scatter_material_grad(const DevScene& S, const Material& mt, uint32_t mat, float u, float v, f3 gA,
+                                    float gR, float gM, GradSink& sink) {
+    mat = min(mat, S.n_materials - 1u);
+    const float4 m0 = S.mats[mat * 4 + 0], m1 = S.mats[mat * 4 + 1], m2 = S.mats[mat * 4 + 2];
+    const int at = __float_as_int(m2.y), rt = __float_as_int(m2.z), tt = __float_as_int(m2.w);
+    uint32_t idx[4];
+    float w[4];
+    if (at >= 0 && S.tex[at].grad) {
+        const TexDesc& t = S.tex[at];
+        tex_coords(t, u, v, idx, w);
+        const f3 g = gA * xyz(m0);
+        for (int k = 0; k < 4; ++k) {
+            if (t.c >= 2) {
+                sink.add(t, (size_t)idx[k] * 4 + 0, w[k] * g.x);
+                sink.add(t, (size_t)idx[k] * 4 + 1, w[k] * g.y);
+                sink.add(t, (size_t)idx[k] * 4 + 2, w[k] * g.z);
+            } else {
+                sink.add(t, idx[k], w[k] * (g.x + g.y + g.z));
+            }
+        }
+    }
+    if (rt >= 0 && S.tex[rt].grad && mt.rough_raw > 0.04f && mt.rough_raw < 1.0f) {
+        const TexDesc& t = S.tex[rt];
+        tex_coords(t, u, v, idx, w);
+        const float g = gR * m0.w;
+        for (int k = 0; k < 4; ++k) sink.add(t, (size_t)idx[k] * (t.c >= 2 ? 4 : 1), w[k] * g);
+    }
+    if (tt >= 0 && S.tex[tt].grad && mt.metal_raw > 0.0f && mt.metal_raw < 1.0f) {
+        const TexDesc& t = S.tex[tt];
+        tex_coords(t, u, v, idx, w);
+        const float g = gM * m1.x;
+        for (int k = 0; k < 4; ++k) sink.add(t, (size_t)idx[k] * (t.c >= 2 ? 4 : 1), w[k] * g);
+    }
+}
 
 // ------------------------------------------------------------------ analytic lights
 // Reference light formulas, pt_raygen_offline.rgen:311-389 (Le = color*I/max(4*3.14159*r^2, 0.01)).

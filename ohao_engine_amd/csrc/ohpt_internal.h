@@ -37,23 +37,30 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t
 // ---- traversal (traverse.hip) ---------------------------------------------
 constexpr uint32_t kTraceBlock = 128;
 constexpr uint32_t kStackLds = 32;       // per-thread LDS stack entries
-constexpr uint32_t kStackOvf = 32;       // per-ray global overflow entries
+constexpr uint32_t kStackOvf = 32;       // per-thread global overflow entries
 constexpr uint32_t kMaxTreeDepth = 60;   // register(4) + LDS + overflow comfortably above this
 
+// Persistent-thread grid of the traversal kernels and the size of their
+// global stack-overflow buffer (one kStackOvf slot per resident thread).
+uint32_t trace_grid_blocks();
+size_t trace_overflow_bytes();
+
 // Closest hit for queued paths: ray of path queue[j] -> hits[j] = (t, slot|miss, u, v).
+// `fetch` is a zeroed counter used for dynamic ray fetching.
 void launch_extend(const DevScene& S, const uint32_t* queue, const uint32_t* count, uint32_t max_count,
-                   const float4* ray_o, const float4* ray_d, float4* hits, uint32_t* ovf,
+                   const float4* ray_o, const float4* ray_d, float4* hits, uint32_t* fetch, uint32_t* ovf,
                    unsigned long long* stats, hipStream_t s);
 // Shadow rays: sh_o[j] = (o, tmax), sh_d[j] = (d, pid*2+k bits), sh_c[j] = contribution;
-// unoccluded ones add sh_c into ln[k][pid].
+// unoccluded ones add sh_c into ln[k][pid] and, when `vis` is given, set bit
+// 2*bounce+k of vis[pid] (for path replay).
 void launch_connect(const DevScene& S, const uint32_t* count, uint32_t max_count, const float4* sh_o,
-                    const float4* sh_d, const float4* sh_c, float4* ln0, float4* ln1, uint32_t* ovf,
-                    hipStream_t s);
+                    const float4* sh_d, const float4* sh_c, float4* ln0, float4* ln1, uint32_t* fetch,
+                    uint32_t* ovf, uint32_t* vis, uint32_t bounce, hipStream_t s);
 // Caller rays (ox,oy,oz,tmin,dx,dy,dz,tmax) -> (t, prim|miss, u, v) / occluded flags.
-void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* ovf,
-                       hipStream_t s);
-void launch_occluded_rays(const DevScene& S, const float4* rays, uint32_t* out, uint32_t n, uint32_t* ovf,
-                          hipStream_t s);
+void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* fetch,
+                       uint32_t* ovf, hipStream_t s);
+void launch_occluded_rays(const DevScene& S, const float4* rays, uint32_t* out, uint32_t n, uint32_t* fetch,
+                          uint32_t* ovf, hipStream_t s);
 
 // ---- integrator (integrator.hip) ------------------------------------------
 struct FrameParams {
@@ -79,14 +86,25 @@ struct PathState {
     float4* sh_o;
     float4* sh_d;
     float4* sh_c;
-    uint32_t* counters;   // [0..kMaxBounce+1]: extend counts; [64..]: shadow counts
+    uint32_t* counters;   // kCounterWords: extend counts, shadow counts, fetch counters
     uint32_t capacity;
 };
-constexpr uint32_t kShadowCounterBase = 64;
+constexpr uint32_t kShadowCounterBase = 64;       // [64 + b]: shadow rays queued at bounce b
+constexpr uint32_t kExtendFetchBase = 128;        // [128 + b]: dynamic-fetch counter of extend b
+constexpr uint32_t kConnectFetchBase = 192;       // [192 + b]: dynamic-fetch counter of connect b
+constexpr uint32_t kCounterWords = 256;
 
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_generate(const FrameParams& f, const PathState& ps, hipStream_t s);
 void launch_shade(const DevScene& S, const FrameParams& f, const PathState& ps, uint32_t bounce, hipStream_t s);
+// Path-replay adjoint of one shading step (same random numbers, recorded visibility).
+void launch_shade_adjoint(const DevScene& S, const FrameParams& f, const PathState& ps, uint32_t bounce,
+                          const uint32_t* vis, float4* lrem, const float4* dl_dimage, float inv_spp,
+                          uint32_t* rec_key, uint32_t* rec_val, hipStream_t s);
+// Deterministic mode: sums runs of equal keys (sorted records) into arena[key].
+void launch_segment_reduce(const uint32_t* keys, const uint32_t* vals, uint32_t n, float* arena, hipStream_t s);
+// lrem[pid] = total radiance of path pid from the forward pass
+void launch_adjoint_init(const PathState& ps, uint32_t n, float4* lrem, hipStream_t s);
 void launch_accumulate(const FrameParams& f, const PathState& ps, float4* accum, float4* moments,
                        float4* aov_albedo, float4* aov_normal, hipStream_t s);
 void launch_resolve(const float4* accum, float4* out, uint32_t n, hipStream_t s);

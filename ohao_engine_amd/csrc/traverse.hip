@@ -1,9 +1,15 @@
 // Extend (closest hit) and connect (any hit) kernels.
 //
+// Persistent threads: the grid fills the GPU once; each wave64 lane runs a
+// traversal state machine and, whenever its ray finishes, the lanes that went
+// idle fetch new ray indices with one wave-aggregated atomic (dynamic ray
+// fetching, Aila & Laine 2009) — so lanes do not wait for the longest ray of a
+// batch and no blocks are launched for rays that do not exist.
+//
 // Short-stack traversal of the 64-byte two-child LBVH nodes: the top four
 // stack entries live in VGPRs (shifted on push/pop), deeper entries spill to a
 // per-thread column of LDS ([depth][thread], bank-conflict free), and beyond
-// that to a per-ray slot in global memory. Leaves (one triangle each) are
+// that to a per-thread slot in global memory. Leaves (one triangle each) are
 // intersected as soon as their box is hit, near child first; the visiting
 // order and every FP operation match the sequential restatement in the
 // oracle, so hits agree bit for bit.
@@ -19,6 +25,8 @@ namespace ohpt {
 namespace {
 
 constexpr float kGamma3Scale = 1.0f + 2.0f * (3.0f * 5.9604645e-08f) / (1.0f - 3.0f * 5.9604645e-08f);
+// lanes keep stepping without refilling while at least this many are busy
+constexpr int kRefillThreshold = 48;
 
 struct RayPre {
     float ox, oy, oz;
@@ -102,9 +110,8 @@ struct Stack {
     uint32_t r0, r1, r2, r3;
     int nreg;     // entries held in registers (top of stack)
     int nmem;     // entries below them (LDS, then overflow)
-    uint32_t* lds;     // column base: lds[k * kTraceBlock]
-    uint32_t* ovf;     // per-ray overflow base
-    __device__ __forceinline__ void push(uint32_t x) {
+    __device__ __forceinline__ void reset() { nreg = 0; nmem = 0; }
+    __device__ __forceinline__ void push(uint32_t x, uint32_t* lds, uint32_t* ovf) {
         if (nreg == 4) {
             if (nmem < (int)kStackLds) lds[nmem * kTraceBlock] = r3;
             else ovf[nmem - (int)kStackLds] = r3;
@@ -115,7 +122,7 @@ struct Stack {
         ++nreg;
     }
     __device__ __forceinline__ bool empty() const { return nreg == 0 && nmem == 0; }
-    __device__ __forceinline__ uint32_t pop() {
+    __device__ __forceinline__ uint32_t pop(const uint32_t* lds, const uint32_t* ovf) {
         if (nreg > 0) {
             const uint32_t x = r0;
             r0 = r1; r1 = r2; r2 = r3;
@@ -127,175 +134,261 @@ struct Stack {
     }
 };
 
-template <bool kAnyHit>
-__device__ __forceinline__ float4 traverse(const DevScene& S, float3 o, float3 d, float tmin, float tmax,
-                                           uint32_t* lds_col, uint32_t* ovf, uint32_t& visits,
-                                           uint32_t& tests) {
-    float4 hit = make_float4(tmax, __uint_as_float(kMiss), 0.0f, 0.0f);
-    if (S.n_tris == 0) return hit;
-    const RayPre r = ray_setup(o, d);
+// Ray sources/sinks of the four kernels.
+struct ExtendIO {
+    const uint32_t* queue;
+    const float4* ray_o;
+    const float4* ray_d;
+    float4* hits;
+    __device__ __forceinline__ void load(uint32_t j, float3& o, float3& d, float& tmin, float& tmax) const {
+        const uint32_t pid = queue[j];
+        const float4 a = ray_o[pid], b = ray_d[pid];
+        o = make_float3(a.x, a.y, a.z);
+        d = make_float3(b.x, b.y, b.z);
+        tmin = 0.0f;
+        tmax = INFINITY;
+    }
+    __device__ __forceinline__ void store(const DevScene&, uint32_t j, float4 h) const { hits[j] = h; }
+};
+
+struct ConnectIO {
+    const float4* sh_o;
+    const float4* sh_d;
+    const float4* sh_c;
+    float4* ln0;
+    float4* ln1;
+    uint32_t* vis;        // optional: per-path visibility bits for path replay
+    uint32_t bounce;
+    __device__ __forceinline__ void load(uint32_t j, float3& o, float3& d, float& tmin, float& tmax) const {
+        const float4 a = sh_o[j], b = sh_d[j];
+        o = make_float3(a.x, a.y, a.z);
+        d = make_float3(b.x, b.y, b.z);
+        tmin = 0.0f;
+        tmax = a.w;
+    }
+    __device__ __forceinline__ void store(const DevScene&, uint32_t j, float4 h) const {
+        if (__float_as_uint(h.y) != kMiss) return;
+        const uint32_t tag = __float_as_uint(sh_d[j].w);
+        const uint32_t pid = tag >> 1;
+        float4* dst = (tag & 1u) ? ln1 : ln0;
+        const float4 c = sh_c[j];
+        float4 v = dst[pid];
+        v.x += c.x; v.y += c.y; v.z += c.z;
+        dst[pid] = v;
+        if (vis) atomicOr(&vis[pid], 1u << (2u * bounce + (tag & 1u)));
+    }
+};
+
+struct RaysIO {
+    const float4* rays;
+    float4* hits;
+    uint32_t* occl;
+    __device__ __forceinline__ void load(uint32_t j, float3& o, float3& d, float& tmin, float& tmax) const {
+        const float4 a = rays[(size_t)j * 2], b = rays[(size_t)j * 2 + 1];
+        o = make_float3(a.x, a.y, a.z);
+        d = make_float3(b.x, b.y, b.z);
+        tmin = a.w;
+        tmax = b.w;
+    }
+    __device__ __forceinline__ void store(const DevScene& S, uint32_t j, float4 h) const {
+        const uint32_t s = __float_as_uint(h.y);
+        if (occl) {
+            occl[j] = s != kMiss ? 1u : 0u;
+        } else {
+            if (s != kMiss) h.y = S.tris[(size_t)s * 3].w;   // slot -> caller's triangle id
+            hits[j] = h;
+        }
+    }
+};
+
+template <bool kAnyHit, class IO>
+__device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io, uint32_t count,
+                                                 uint32_t* __restrict__ fetch, uint32_t* lds_col, uint32_t* ovf,
+                                                 unsigned long long* stats) {
+    const uint32_t lane = lane_id();
+    RayPre r;
     Stack st;
     st.r0 = st.r1 = st.r2 = st.r3 = 0;
-    st.nreg = 0; st.nmem = 0;
-    st.lds = lds_col; st.ovf = ovf;
-    uint32_t cur = 0;
-    float tclose = tmax;
+    st.reset();
+    uint32_t cur = 0, j = 0;
+    float tmin = 0.0f, tclose = 0.0f;
+    float4 hit = make_float4(0, 0, 0, 0);
+    bool active = false, drained = false;
     uint32_t nv = 0, nt = 0;
     for (;;) {
-        ++nv;
-        const float4* np = S.nodes + (size_t)cur * 4;
-        const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
-        const float e0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tmin, tclose);
-        const float e1 = box_hit(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tmin, tclose);
-        const uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
-        const bool h0 = e0 != INFINITY, h1 = e1 != INFINITY;
-        const bool sw = h1 && (!h0 || e1 < e0);
-        const uint32_t A = sw ? c1 : c0, B = sw ? c0 : c1;
-        const bool hA = sw ? h1 : h0, hB = sw ? h0 : h1;
-        const bool lA = hA && (A & kLeafBit), lB = hB && (B & kLeafBit);
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const bool test = k == 0 ? lA : lB;
-            if (test) {
-                const uint32_t s = (k == 0 ? A : B) & ~kLeafBit;
-                const float4* tp = S.tris + (size_t)s * 3;
-                const float4 ta = tp[0], tb = tp[1], tc = tp[2];
-                float tt, uu, vv;
-                ++nt;
-                if (tri_hit(r, ta, tb, tc, tmin, tclose, tt, uu, vv)) {
-                    tclose = tt;
-                    hit = make_float4(tt, __uint_as_float(s), uu, vv);
-                    if (kAnyHit) {
-                        visits = nv;
-                        tests = nt;
-                        return hit;
+        // ---- refill idle lanes with new rays (one atomic per wave)
+        if (!drained) {
+            const uint64_t want = __ballot(!active);
+            if (want != 0) {
+                const int leader = __ffsll((unsigned long long)want) - 1;
+                const uint32_t nw = (uint32_t)__popcll(want);
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(fetch, nw);
+                base = __shfl(base, leader);
+                if (base + nw >= count) drained = true;
+                if (!active) {
+                    j = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
+                    if (j < count) {
+                        float3 o, d;
+                        float tmax;
+                        io.load(j, o, d, tmin, tmax);
+                        r = ray_setup(o, d);
+                        tclose = tmax;
+                        hit = make_float4(tmax, __uint_as_float(kMiss), 0.0f, 0.0f);
+                        cur = 0;
+                        st.reset();
+                        active = S.n_tris > 0;
+                        if (!active) io.store(S, j, hit);
                     }
                 }
             }
         }
-        const bool iA = hA && !(A & kLeafBit), iB = hB && !(B & kLeafBit);
-        if (iA) {
-            cur = A;
-            if (iB) st.push(B);
-        } else if (iB) {
-            cur = B;
-        } else {
-            if (st.empty()) break;
-            cur = st.pop();
+        if (__ballot(active) == 0) {
+            if (drained) break;
+            continue;
         }
-    }
-    visits = nv;
-    tests = nt;
-    return hit;
-}
-
-__global__ __launch_bounds__(kTraceBlock) void k_extend(DevScene S, const uint32_t* __restrict__ queue,
-                                                        const uint32_t* __restrict__ count,
-                                                        const float4* __restrict__ ray_o,
-                                                        const float4* __restrict__ ray_d,
-                                                        float4* __restrict__ hits, uint32_t* __restrict__ ovf,
-                                                        unsigned long long* __restrict__ stats) {
-    __shared__ uint32_t lds[kStackLds * kTraceBlock];
-    const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
-    const bool active = j < *count;
-    uint32_t nv = 0, nt = 0;
-    if (active) {
-        const uint32_t pid = queue[j];
-        const float4 o = ray_o[pid], d = ray_d[pid];
-        hits[j] = traverse<false>(S, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), 0.0f, INFINITY,
-                                  lds + threadIdx.x, ovf + (size_t)j * kStackOvf, nv, nt);
+        // ---- traverse until too many lanes have finished (or to the end once drained)
+        do {
+            if (active) {
+                ++nv;
+                const float4* np = S.nodes + (size_t)cur * 4;
+                const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+                const float e0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tmin, tclose);
+                const float e1 = box_hit(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tmin, tclose);
+                const uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
+                const bool h0 = e0 != INFINITY, h1 = e1 != INFINITY;
+                const bool sw = h1 && (!h0 || e1 < e0);
+                const uint32_t A = sw ? c1 : c0, B = sw ? c0 : c1;
+                const bool hA = sw ? h1 : h0, hB = sw ? h0 : h1;
+                const bool lA = hA && (A & kLeafBit), lB = hB && (B & kLeafBit);
+                bool done = false;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const bool test = (k == 0 ? lA : lB) && !done;
+                    if (test) {
+                        const uint32_t s = (k == 0 ? A : B) & ~kLeafBit;
+                        const float4* tp = S.tris + (size_t)s * 3;
+                        const float4 ta = tp[0], tb = tp[1], tc = tp[2];
+                        float tt, uu, vv;
+                        ++nt;
+                        if (tri_hit(r, ta, tb, tc, tmin, tclose, tt, uu, vv)) {
+                            tclose = tt;
+                            hit = make_float4(tt, __uint_as_float(s), uu, vv);
+                            if (kAnyHit) done = true;
+                        }
+                    }
+                }
+                if (!done) {
+                    const bool iA = hA && !(A & kLeafBit), iB = hB && !(B & kLeafBit);
+                    if (iA) {
+                        cur = A;
+                        if (iB) st.push(B, lds_col, ovf);
+                    } else if (iB) {
+                        cur = B;
+                    } else if (st.empty()) {
+                        done = true;
+                    } else {
+                        cur = st.pop(lds_col, ovf);
+                    }
+                }
+                if (done) {
+                    io.store(S, j, hit);
+                    active = false;
+                }
+            }
+        } while (__popcll(__ballot(active)) >= (drained ? 1 : kRefillThreshold));
     }
     if (stats) {
-        // node visits and triangle tests of this launch: wave sums, one atomic per wave
         for (int off = 32; off > 0; off >>= 1) {
             nv += __shfl_xor(nv, off);
             nt += __shfl_xor(nt, off);
         }
-        if ((threadIdx.x & 63u) == 0) {
+        if (lane == 0) {
             atomicAdd(&stats[0], (unsigned long long)nv);
             atomicAdd(&stats[1], (unsigned long long)nt);
         }
     }
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_connect(DevScene S, const uint32_t* __restrict__ count,
-                                                         const float4* __restrict__ sh_o,
-                                                         const float4* __restrict__ sh_d,
-                                                         const float4* __restrict__ sh_c, float4* __restrict__ ln0,
-                                                         float4* __restrict__ ln1, uint32_t* __restrict__ ovf) {
+__global__ __launch_bounds__(kTraceBlock) void k_extend(DevScene S, ExtendIO io, const uint32_t* __restrict__ count,
+                                                        uint32_t* __restrict__ fetch, uint32_t* __restrict__ ovf,
+                                                        unsigned long long* __restrict__ stats) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
-    const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (j >= *count) return;
-    const float4 o = sh_o[j], d = sh_d[j];
-    uint32_t nv = 0, nt = 0;
-    const float4 h = traverse<true>(S, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), 0.0f, o.w,
-                                    lds + threadIdx.x, ovf + (size_t)j * kStackOvf, nv, nt);
-    if (__float_as_uint(h.y) != kMiss) return;
-    const uint32_t tag = __float_as_uint(d.w);
-    const uint32_t pid = tag >> 1;
-    float4* dst = (tag & 1u) ? ln1 : ln0;
-    const float4 c = sh_c[j];
-    float4 v = dst[pid];
-    v.x += c.x; v.y += c.y; v.z += c.z;
-    dst[pid] = v;
+    const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
+    persistent_trace<false>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, stats);
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DevScene S, const float4* __restrict__ rays,
-                                                            float4* __restrict__ hits, uint32_t n,
-                                                            uint32_t* __restrict__ ovf) {
+__global__ __launch_bounds__(kTraceBlock) void k_connect(DevScene S, ConnectIO io, const uint32_t* __restrict__ count,
+                                                         uint32_t* __restrict__ fetch, uint32_t* __restrict__ ovf) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
-    const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (j >= n) return;
-    const float4 o = rays[(size_t)j * 2], d = rays[(size_t)j * 2 + 1];
-    uint32_t nv = 0, nt = 0;
-    float4 h = traverse<false>(S, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), o.w, d.w,
-                               lds + threadIdx.x, ovf + (size_t)j * kStackOvf, nv, nt);
-    const uint32_t s = __float_as_uint(h.y);
-    if (s != kMiss) h.y = S.tris[(size_t)s * 3].w;   // slot -> caller's triangle id
-    hits[j] = h;
+    const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
+    persistent_trace<true>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_occluded_rays(DevScene S, const float4* __restrict__ rays,
-                                                               uint32_t* __restrict__ out, uint32_t n,
-                                                               uint32_t* __restrict__ ovf) {
+template <bool kAnyHit>
+__global__ __launch_bounds__(kTraceBlock) void k_rays(DevScene S, RaysIO io, uint32_t n, uint32_t* __restrict__ fetch,
+                                                      uint32_t* __restrict__ ovf) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
-    const uint32_t j = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (j >= n) return;
-    const float4 o = rays[(size_t)j * 2], d = rays[(size_t)j * 2 + 1];
-    uint32_t nv = 0, nt = 0;
-    const float4 h = traverse<true>(S, make_float3(o.x, o.y, o.z), make_float3(d.x, d.y, d.z), o.w, d.w,
-                                    lds + threadIdx.x, ovf + (size_t)j * kStackOvf, nv, nt);
-    out[j] = __float_as_uint(h.y) != kMiss ? 1u : 0u;
+    const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
+    persistent_trace<kAnyHit>(S, io, n, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
 }
 
-inline uint32_t blocks_for(uint32_t n) { return (n + kTraceBlock - 1) / kTraceBlock; }
+__global__ void k_zero(uint32_t* p) { *p = 0u; }
+
+uint32_t g_grid = 0;
 
 }  // namespace
 
+uint32_t trace_grid_blocks() {
+    if (g_grid == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend, kTraceBlock, 0);
+        if (cus <= 0) cus = 256;
+        if (per_cu <= 0) per_cu = 8;
+        g_grid = (uint32_t)(cus * per_cu);
+    }
+    return g_grid;
+}
+
+size_t trace_overflow_bytes() { return (size_t)trace_grid_blocks() * kTraceBlock * kStackOvf * 4; }
+
 void launch_extend(const DevScene& S, const uint32_t* queue, const uint32_t* count, uint32_t max_count,
-                   const float4* ray_o, const float4* ray_d, float4* hits, uint32_t* ovf,
+                   const float4* ray_o, const float4* ray_d, float4* hits, uint32_t* fetch, uint32_t* ovf,
                    unsigned long long* stats, hipStream_t s) {
     if (max_count == 0) return;
-    k_extend<<<blocks_for(max_count), kTraceBlock, 0, s>>>(S, queue, count, ray_o, ray_d, hits, ovf, stats);
+    ExtendIO io{queue, ray_o, ray_d, hits};
+    const uint32_t grid = std::min(trace_grid_blocks(), (max_count + kTraceBlock - 1) / kTraceBlock);
+    k_extend<<<grid, kTraceBlock, 0, s>>>(S, io, count, fetch, ovf, stats);
 }
 
 void launch_connect(const DevScene& S, const uint32_t* count, uint32_t max_count, const float4* sh_o,
-                    const float4* sh_d, const float4* sh_c, float4* ln0, float4* ln1, uint32_t* ovf,
-                    hipStream_t s) {
+                    const float4* sh_d, const float4* sh_c, float4* ln0, float4* ln1, uint32_t* fetch,
+                    uint32_t* ovf, uint32_t* vis, uint32_t bounce, hipStream_t s) {
     if (max_count == 0) return;
-    k_connect<<<blocks_for(max_count), kTraceBlock, 0, s>>>(S, count, sh_o, sh_d, sh_c, ln0, ln1, ovf);
+    ConnectIO io{sh_o, sh_d, sh_c, ln0, ln1, vis, bounce};
+    const uint32_t grid = std::min(trace_grid_blocks(), (max_count + kTraceBlock - 1) / kTraceBlock);
+    k_connect<<<grid, kTraceBlock, 0, s>>>(S, io, count, fetch, ovf);
 }
 
-void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* ovf,
-                       hipStream_t s) {
+void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* fetch,
+                       uint32_t* ovf, hipStream_t s) {
     if (n == 0) return;
-    k_trace_rays<<<blocks_for(n), kTraceBlock, 0, s>>>(S, rays, hits, n, ovf);
+    k_zero<<<1, 1, 0, s>>>(fetch);
+    RaysIO io{rays, hits, nullptr};
+    const uint32_t grid = std::min(trace_grid_blocks(), (n + kTraceBlock - 1) / kTraceBlock);
+    k_rays<false><<<grid, kTraceBlock, 0, s>>>(S, io, n, fetch, ovf);
 }
 
-void launch_occluded_rays(const DevScene& S, const float4* rays, uint32_t* out, uint32_t n, uint32_t* ovf,
-                          hipStream_t s) {
+void launch_occluded_rays(const DevScene& S, const float4* rays, uint32_t* out, uint32_t n, uint32_t* fetch,
+                          uint32_t* ovf, hipStream_t s) {
     if (n == 0) return;
-    k_occluded_rays<<<blocks_for(n), kTraceBlock, 0, s>>>(S, rays, out, n, ovf);
+    k_zero<<<1, 1, 0, s>>>(fetch);
+    RaysIO io{rays, nullptr, out};
+    const uint32_t grid = std::min(trace_grid_blocks(), (n + kTraceBlock - 1) / kTraceBlock);
+    k_rays<true><<<grid, kTraceBlock, 0, s>>>(S, io, n, fetch, ovf);
 }
 
 }  // namespace ohpt

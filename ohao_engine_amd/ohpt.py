@@ -79,6 +79,9 @@ SIGNATURES = [
     ("ohpt_read_moments", C.c_int, [_vp, _f32p, _f32p, C.c_int]),
     ("ohpt_read_pixels", C.c_int, [_vp, _u8p, C.c_int]),
     ("ohpt_read_aovs", C.c_int, [_vp, _f32p, _f32p, _f32p, C.c_int]),
+    ("ohpt_adjoint", C.c_int, [_vp, _f32p, C.c_int, C.c_uint32, C.c_uint32]),
+    ("ohpt_zero_gradients", C.c_int, [_vp]),
+    ("ohpt_read_texture_gradient", C.c_int, [_vp, C.c_int32, _f32p, C.c_int]),
     ("ohpt_trace_rays", C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_int]),
     ("ohpt_occluded_rays", C.c_int, [_vp, _f32p, _u32p, C.c_uint32, C.c_int]),
 ]
@@ -306,6 +309,26 @@ class Context:
         d = np.zeros((self.height, self.width), np.float32)
         _check(self.L.ohpt_read_aovs(self.h, _fp(a), _fp(n), _fp(d), 0), "ohpt_read_aovs")
         return a, n, d
+
+    # -- differentiation ---------------------------------------------------------
+    def zero_gradients(self):
+        _check(self.L.ohpt_zero_gradients(self.h), "ohpt_zero_gradients")
+
+    def adjoint(self, dl_dimage, spp: int, deterministic: bool = False):
+        """dl_dimage: (H, W, 3|4) dL/d(mean radiance); accumulates texel gradients."""
+        flags = 1 if deterministic else 0
+        if hasattr(dl_dimage, "data_ptr"):
+            _check(self.L.ohpt_adjoint(self.h, C.cast(dl_dimage.data_ptr(), _f32p), 1, spp, flags), "ohpt_adjoint")
+            return
+        g = np.zeros((self.height, self.width, 4), np.float32)
+        d = np.asarray(dl_dimage, np.float32)
+        g[..., :min(d.shape[-1], 4)] = d[..., :4]
+        _check(self.L.ohpt_adjoint(self.h, _fp(g), 0, spp, flags), "ohpt_adjoint")
+
+    def texture_gradient(self, tid: int, shape) -> np.ndarray:
+        out = np.zeros(shape, np.float32)
+        _check(self.L.ohpt_read_texture_gradient(self.h, tid, _fp(out), 0), "ohpt_read_texture_gradient")
+        return out
 
     def trace_rays(self, rays: np.ndarray) -> np.ndarray:
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

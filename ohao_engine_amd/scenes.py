@@ -353,7 +353,16 @@ def procedural(width=1920, height=1080, seed=7, sphere_freq=40, hf_cells=(1024, 
     pos_l, nrm_l, uv_l, mat_l = [], [], [], []
     p, n, u = flatten(Vh, Nh, UVh, Fh)
     pos_l.append(p); nrm_l.append(n); uv_l.append(u); mat_l.append(np.zeros(p.shape[0], np.uint32))
-    mats = [material((0.5, 0.5, 0.5), roughness=0.9, metallic=0.0)]
+    # ground albedo texture (1024x512 RGB value noise in [0.35, 0.65]): the
+    # differentiable parameter of the adjoint measured on this scene
+    tw, th = hf_cells
+    gx, gz = np.meshgrid(np.arange(tw) / tw * 16.0, np.arange(th) / th * 8.0, indexing="xy")
+    gp = np.stack([gx.ravel(), gz.ravel(), np.full(gx.size, 0.5)], axis=1)
+    g = 0.5 + 0.15 * fbm3(gp, seed + 2, octaves=3, base_freq=1.0)
+    ground_tex = np.repeat(g.reshape(th, tw, 1), 3, axis=2).astype(np.float32)
+    ground_tex[..., 0] *= 1.05
+    ground_tex[..., 2] *= 0.95
+    mats = [material((1.0, 1.0, 1.0), roughness=0.9, metallic=0.0, albedo_tex=0)]
     side = int(round(math.sqrt(n_instances)))
     for k in range(n_instances):
         i, j = k % side, k // side
@@ -373,7 +382,7 @@ def procedural(width=1920, height=1080, seed=7, sphere_freq=40, hf_cells=(1024, 
     cam = Camera.look_at((0.0, 7.0, 21.0), (0.0, 0.0, -1.0), 50.0)
     return Scene(pos=np.concatenate(pos_l), nrm=np.concatenate(nrm_l), uv=np.concatenate(uv_l),
                  mat=np.concatenate(mat_l), materials=np.array(mats, MATERIAL_DTYPE), env=env,
-                 camera=cam, width=width, height=height, name="procedural",
+                 textures=[ground_tex], camera=cam, width=width, height=height, name="procedural",
                  settings=Settings(max_bounces=4, rr_start=3, sampler=SAMPLER_PCG32, seed=seed, spp=1))
 
 

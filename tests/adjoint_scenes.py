@@ -1,0 +1,50 @@
+"""Small textured scenes for gradient tests (albedo RGB, roughness and
+metallic textures on a displaced sphere, environment + rectangular light)."""
+import numpy as np
+
+from ohao_engine_amd import scenes as S
+
+
+def textured_probe(width=32, height=32, freq=10, tex=16, rough_metal=True, seed=5, rr_start=1000, max_bounces=3):
+    rng = np.random.default_rng(seed)
+    V, N, UV, F = S.displaced_sphere(freq, 0.05, seed)
+    p, n, u = S.flatten(V, N, UV, F, 1.0, 0.0, (0.0, 1.0, 0.0))
+    P = lambda x, y, z: np.array([x, y, z], np.float64)
+    plane = np.array(S._quad(P(-4, -0.1, -4), P(-4, -0.1, 4), P(4, -0.1, 4), P(4, -0.1, -4)), np.float32)
+    pos = np.concatenate([p, plane])
+    nrm = np.concatenate([n, np.tile(np.array([0, 1, 0] * 3, np.float32), (2, 1))])
+    uv = np.concatenate([u, np.zeros((2, 6), np.float32)])
+    mat = np.concatenate([np.zeros(p.shape[0], np.uint32), np.ones(2, np.uint32)])
+    textures = [rng.uniform(0.2, 0.8, (tex, tex, 3)).astype(np.float32)]
+    if rough_metal:
+        textures.append(rng.uniform(0.25, 0.75, (tex // 2, tex // 2, 1)).astype(np.float32))
+        textures.append(rng.uniform(0.1, 0.9, (tex // 2, tex // 2, 1)).astype(np.float32))
+        m0 = S.material((1, 1, 1), roughness=1.0, metallic=1.0, albedo_tex=0, roughness_tex=1, metallic_tex=2)
+    else:
+        m0 = S.material((1, 1, 1), roughness=0.5, metallic=0.0, albedo_tex=0)
+    mats = np.array([m0, S.material((0.6, 0.6, 0.6), roughness=0.8)], S.MATERIAL_DTYPE)
+    light = np.zeros((1, 20), np.float32)
+    light[0, 0:4] = (-1.0, 4.0, -1.0, S.LIGHT_AREA_RECT)
+    light[0, 4:8] = (1.0, 0.97, 0.9, 4.0)
+    light[0, 12:16] = (2.0, 0.0, 0.0, 0.0)
+    light[0, 16:20] = (0.0, 0.0, 2.0, 4.0)
+    cam = S.Camera.look_at((2.2, 2.0, 2.6), (0.0, 1.0, 0.0), 40.0)
+    return S.Scene(pos=pos, nrm=nrm, uv=uv, mat=mat, materials=mats, textures=textures, lights=light,
+                   env=S.sky_env(64, 32, low=0.1, high=0.7), camera=cam, width=width, height=height,
+                   settings=S.Settings(max_bounces=max_bounces, rr_start=rr_start, seed=3, spp=16))
+
+
+def oracle_detached_fd(oscene, sc, weights, spp, tid, texel, channel, eps=1e-3):
+    """Central FD of L = sum(weights * mean image) with the evaluation-side
+    texture perturbed (sampling side fixed, same random numbers), FP64."""
+    base = sc.textures[tid]
+    out, vals = [], []
+    for sgn in (+1.0, -1.0):
+        t = base.copy()
+        t[texel[0], texel[1], channel] = np.float32(float(base[texel[0], texel[1], channel]) + sgn * eps)
+        vals.append(float(t[texel[0], texel[1], channel]))   # the float32 value actually used
+        oscene.set_texture_eval(tid, t)
+        s, _ = oscene.render(spp, sc.settings.seed, precision=64)
+        out.append(float(np.sum(weights * s) / spp))
+    oscene.set_texture_eval(tid, None)
+    return (out[0] - out[1]) / (vals[0] - vals[1])
