@@ -112,12 +112,13 @@ def main():
     value = paths / wall / 1e6
     byts = extend_bytes(acc["rays_extend"], acc["node_visits"], acc["tri_tests"])
     achieved = byts / (acc["ms_extend"] * 1e-3) / 1e9 if acc["ms_extend"] > 0 else 0.0
-    traffic = None
+    traffic, pmc = None, {}
     if TRAFFIC_FILE.exists():
         try:
-            traffic = json.loads(TRAFFIC_FILE.read_text()).get("bytes_per_launch")
+            pmc = json.loads(TRAFFIC_FILE.read_text())
+            traffic = pmc.get("bytes_per_launch")
         except Exception:
-            traffic = None
+            traffic, pmc = None, {}
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "k_extend (closest-hit LBVH traversal)",
@@ -125,7 +126,9 @@ def main():
                 "avg_launch_ms": round(acc["ms_extend"] / max(acc["extend_launches"], 1), 5),
                 "algorithmic_bytes_per_launch": round(byts / max(acc["extend_launches"], 1)),
                 "node_visits_per_ray": round(acc["node_visits"] / max(acc["rays_extend"], 1), 3),
-                "tri_tests_per_ray": round(acc["tri_tests"] / max(acc["rays_extend"], 1), 3)}
+                "tri_tests_per_ray": round(acc["tri_tests"] / max(acc["rays_extend"], 1), 3),
+                "l2_hit_pct": pmc.get("l2_hit_pct"), "occupancy_pct_first_bounce": pmc.get("occupancy_pct_first_bounce"),
+                "traffic_source": "profiles/" + pmc["source"] if pmc.get("source") else None}
     breakdown = {k: round(acc[k] / a.steps, 4) for k in ("ms_total", "ms_extend", "ms_shade", "ms_connect")}
     breakdown["extend_rays_per_step"] = acc["rays_extend"] // a.steps
     breakdown["shadow_rays_per_step"] = acc["rays_shadow"] // a.steps
