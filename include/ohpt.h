@@ -250,6 +250,75 @@ int ohpt_trace_rays(ohpt_context* ctx, const float* rays, float* hits, uint32_t 
 /* Occlusion query: out[i] = 1 if any hit in (tmin, tmax). */
 int ohpt_occluded_rays(ohpt_context* ctx, const float* rays, uint32_t* out, uint32_t n, int on_device);
 
+/* ---- learned denoiser ----------------------------------------------------------
+ * Kernel-predicting U-Net: 10 input channels (log1p radiance, albedo, normal,
+ * log1p depth) in bfloat16, five levels of 3x3 convolutions with 32/64/96/128/160
+ * channels, 2x2 max-pool down, bilinear x2 up with skip concatenation, and a
+ * head predicting per-pixel 5x5 softmax filter weights applied to the noisy
+ * radiance. Convolutions run on the matrix cores in BF16 with FP32 accumulation;
+ * parameters and Adam state are FP32. Images of any size are padded to a
+ * multiple of 16 by edge replication.
+ *
+ * Takes the place of the reference's denoise stage: oidnDenoise
+ * (ohao/render/rt/denoise/oidn_denoise.hpp:14-17, same beauty/albedo/normal
+ * guides), selected through VulkanRenderer::setDenoiseMode
+ * (ohao/gpu/vulkan/renderer.hpp:182) and applied lazily by getPixels
+ * (renderer.hpp:255-258). One denoiser owns one HIP stream; not thread-safe. */
+typedef struct ohpt_denoiser ohpt_denoiser;
+
+/* Guide images of `batch` images, each width*height pixels, row-major.
+ * radiance, albedo, normal: RGBA32F (alpha ignored); depth: F32. albedo,
+ * normal and depth may be NULL (that channel group is then zero). */
+typedef struct ohpt_denoise_inputs {
+    const float* radiance;
+    const float* albedo;
+    const float* normal;
+    const float* depth;
+} ohpt_denoise_inputs;
+
+typedef struct ohpt_denoiser_stats {
+    float ms_total;          /* GPU time of the last infer / train call         */
+    float ms_conv;           /* of which 3x3 convolution kernels (when profiled) */
+    double conv_flops;       /* algorithmic 3x3-convolution FLOPs of that call   */
+    uint32_t padded_width, padded_height, batch, reserved;
+} ohpt_denoiser_stats;
+
+/* Parameters are initialised from `seed` (He-normal convolutions, zero bias,
+ * near-zero head so the untrained filter is a 5x5 box). */
+int ohpt_denoiser_create(ohpt_denoiser** out, int device, uint64_t seed);
+int ohpt_denoiser_destroy(ohpt_denoiser* dn);
+/* FP32 parameters: for each convolution in network order, W[tap][cin][cout]
+ * then bias[cout]. set_params resets the Adam moments and step count. */
+int ohpt_denoiser_param_count(ohpt_denoiser* dn, uint64_t* n);
+int ohpt_denoiser_get_params(ohpt_denoiser* dn, float* out, uint64_t n);
+int ohpt_denoiser_set_params(ohpt_denoiser* dn, const float* in, uint64_t n);
+/* Gradient of the last forward_backward / train_step, same layout. */
+int ohpt_denoiser_get_grads(ohpt_denoiser* dn, float* out, uint64_t n);
+/* Loss = mean over pixels and RGB of |T(out) - T(target)| with
+ * T(x) = (ACES(0.5 x) + 1e-3)^(1/2.2); target RGBA32F. Computes the gradient
+ * of every parameter; *loss receives the loss (host pointer, may be NULL). */
+int ohpt_denoiser_forward_backward(ohpt_denoiser* dn, const ohpt_denoise_inputs* in, const float* target,
+                                   uint32_t batch, uint32_t width, uint32_t height, float* loss, int on_device);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) with the
+ * stored gradient. */
+int ohpt_denoiser_adam_step(ohpt_denoiser* dn, float lr);
+int ohpt_denoiser_train_step(ohpt_denoiser* dn, const ohpt_denoise_inputs* in, const float* target,
+                             uint32_t batch, uint32_t width, uint32_t height, float lr, float* loss, int on_device);
+/* Denoised RGBA32F (alpha = 1) of `batch` images. */
+int ohpt_denoiser_infer(ohpt_denoiser* dn, const ohpt_denoise_inputs* in, float* out, uint32_t batch,
+                        uint32_t width, uint32_t height, int on_device);
+/* Intermediate tensor `index` of the last call as FP32, for tests: 0..26 are
+ * the network activations in evaluation order (NHWC, padded size), 27 the
+ * head logits (32 channels, 25 used), 100+i the gradient of activation i,
+ * 127 the gradient of the logits. n = element count of `out` (host). */
+int ohpt_denoiser_read_tensor(ohpt_denoiser* dn, uint32_t index, float* out, uint64_t n);
+int ohpt_denoiser_get_stats(ohpt_denoiser* dn, ohpt_denoiser_stats* out);
+/* on != 0: time every convolution launch with HIP events (fills ms_conv). */
+int ohpt_denoiser_set_profiling(ohpt_denoiser* dn, int on);
+/* Denoise the context's current accumulation (mean beauty + first-hit AOVs)
+ * into out (width*height RGBA32F of the context). */
+int ohpt_denoise(ohpt_context* ctx, ohpt_denoiser* dn, float* out, int on_device);
+
 #ifdef __cplusplus
 }
 #endif

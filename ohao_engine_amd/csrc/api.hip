@@ -25,40 +25,20 @@ int fail_hip(const char* what, hipError_t e) {
     return OHPT_ERR_HIP;
 }
 
+}  // namespace
+
+namespace ohpt {
+int set_error(int code, const char* msg) { return fail(code, msg); }
+int set_hip_error(const char* what, hipError_t e) { return fail_hip(what, e); }
+}  // namespace ohpt
+
+namespace {
+
 #define HIPCHK(x)                                   \
     do {                                            \
         hipError_t e_ = (x);                        \
         if (e_ != hipSuccess) return fail_hip(#x, e_); \
     } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
-    DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t ensure(size_t n) {
-        if (n <= bytes && p) return hipSuccess;
-        release();
-        if (n == 0) return hipSuccess;
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) bytes = n;
-        else p = nullptr;
-        return e;
-    }
-    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
-};
 
 struct TexStore {
     DevBuf data;
@@ -116,6 +96,7 @@ struct ohpt_context {
     DevBuf grad_arena;                      // dL/d(texel) of all textures, packed like the textures
     std::vector<size_t> grad_off;           // per-texture offset in the arena (floats)
     DevBuf rec_key, rec_val, rec_scratch;   // deterministic-mode gradient records
+    DevBuf dn_guides, dn_out;               // resolved guides + output staging of ohpt_denoise
 
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> ev_pool;
@@ -1048,6 +1029,32 @@ int ohpt_read_aovs(ohpt_context* c, float* albedo, float* normal, float* depth, 
     if (normal) HIPCHK(hipMemcpyAsync(normal, tn, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
     if (depth) HIPCHK(hipMemcpyAsync(depth, td, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+// ---------------------------------------------------------------- denoise
+int ohpt_denoise(ohpt_context* c, ohpt_denoiser* dn, float* out, int on_device) {
+    if (!c || !dn || !out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_denoise: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const uint32_t n = c->W * c->H;
+    HIPCHK(c->dn_guides.ensure((size_t)n * 52));
+    float* rad = c->dn_guides.as<float>();
+    float* alb = rad + (size_t)n * 4;
+    float* nrm = alb + (size_t)n * 4;
+    float* dep = nrm + (size_t)n * 4;
+    launch_resolve(c->d_accum.as<float4>(), reinterpret_cast<float4*>(rad), n, c->stream);
+    launch_resolve_aovs(c->d_accum.as<float4>(), c->d_aov_albedo.as<float4>(), c->d_aov_normal.as<float4>(), alb,
+                        nrm, dep, n, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float* dst = out;
+    if (!on_device) {
+        HIPCHK(c->dn_out.ensure((size_t)n * 16));
+        dst = c->dn_out.as<float>();
+    }
+    const int rc = denoiser_infer_device(dn, rad, alb, nrm, dep, dst, c->W, c->H);
+    if (rc != OHPT_OK) return rc;
+    if (!on_device) HIPCHK(hipMemcpy(out, dst, (size_t)n * 16, hipMemcpyDeviceToHost));
     return OHPT_OK;
 }
 

@@ -1,0 +1,1368 @@
+// Learned kernel-predicting denoiser (include/ohpt.h, "learned denoiser").
+//
+// Network: 5-level U-Net over NHWC bfloat16 activations.
+//   encoder  x0(16) -> a0,e0(32) -pool-> a1,e1(64) -pool-> a2,e2(96) -pool-> a3,e3(128) -pool-> a4,e4(160)
+//   decoder  up(e4)|e3 -> b3,d3(128) ; up(d3)|e2 -> b2,d2(96) ; up(d2)|e1 -> b1,d1(64) ; up(d1)|e0 -> b0,d0(32)
+//   head     d0 -> 25 logits (FP32) -> softmax -> 5x5 filter applied to the noisy radiance
+// Every 3x3 convolution (forward, input gradient and weight gradient) is an
+// implicit GEMM on the gfx950 matrix cores, v_mfma_f32_16x16x32_bf16, FP32
+// accumulation; bias + ReLU are fused into the forward epilogue.
+//
+// Kernel shapes (wave64, 256-thread blocks = 4 waves):
+//   k_conv3x3<CC>  one block = 16x8 output pixels x 32 output channels. Per
+//                  input-channel chunk of CC channels the 18x10 input halo and
+//                  the chunk's packed weights are staged in LDS; each wave
+//                  owns two 16-pixel rows x 32 channels (2x2 MFMA tiles) and
+//                  walks K = 9 taps x CC channels in steps of 32.
+//   k_wgrad<CC>    one block = (pixel-tile range, 32 output channels, CC
+//                  input channels); K = pixels, 32 per MFMA step; per-block
+//                  partial sums go to a buffer that a fixed-order reduction
+//                  sums (deterministic weight gradients).
+// The input gradient reuses k_conv3x3 with the weights flipped and transposed.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ohpt_internal.h"
+
+using namespace ohpt;
+
+namespace {
+
+#define DNCHK(x)                                             \
+    do {                                                     \
+        hipError_t e_ = (x);                                 \
+        if (e_ != hipSuccess) return set_hip_error(#x, e_); \
+    } while (0)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------- bf16 helpers
+__device__ inline uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);   // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                             // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+inline uint16_t f2bf_host(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// ------------------------------------------------------------- 3x3 convolution
+struct ConvArgs {
+    const uint16_t* in0;   // NHWC, c0 channels
+    const uint16_t* in1;   // NHWC, c1 channels (second half of a concatenation) or null
+    int c0, c1;
+    const uint16_t* wp;    // packed weights [cin/CC][KS][np][32]
+    int np;                // output channels, multiple of 32
+    const float* bias;     // [nb] or null
+    int nb;
+    uint16_t* out0;        // channels [0, oc0) with row stride oc0
+    uint16_t* out1;        // channels [oc0, oc0+oc1) with row stride oc1
+    int oc0, oc1;
+    float* outf;           // FP32 output with np channels (replaces out0/out1)
+    int relu;
+    int B, H, W;
+};
+
+template <int CC>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
+    constexpr int KS = (9 * CC + 31) / 32;   // K steps of 32 per input chunk
+    constexpr int XS = CC + 8;               // LDS halo pixel stride (bf16 elements)
+    constexpr int WS = 40;                   // LDS weight row stride
+    constexpr int VPP = CC / 8;              // 16-byte vectors per halo pixel
+    __shared__ __align__(16) uint16_t xs[180 * XS];
+    __shared__ __align__(16) uint16_t ws[KS * 32 * WS];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int ncout = a.np >> 5;
+    const int b = blockIdx.z / ncout, nc = blockIdx.z - b * ncout;
+    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 8;
+    const int nchunk = (a.c0 + a.c1) / CC;
+
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int c = 0; c < nchunk; ++c) {
+        const int cb = c * CC;
+        const uint16_t* src;
+        int cs, co;
+        if (cb < a.c0) { src = a.in0; cs = a.c0; co = cb; }
+        else { src = a.in1; cs = a.c1; co = cb - a.c0; }
+        __syncthreads();
+        for (int i = tid; i < 180 * VPP; i += 256) {
+            const int p = i / VPP, v = i - p * VPP;
+            const int hy = p / 18, hx = p - hy * 18;
+            const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                val = *reinterpret_cast<const uint4*>(src + ((size_t)(b * a.H + gy) * a.W + gx) * cs + co + v * 8);
+            *reinterpret_cast<uint4*>(&xs[p * XS + v * 8]) = val;
+        }
+        const uint16_t* wsrc = a.wp + ((size_t)c * KS * a.np + (size_t)nc * 32) * 32;
+        for (int i = tid; i < KS * 128; i += 256) {
+            const int s = i >> 7, n = (i >> 2) & 31, v = i & 3;
+            *reinterpret_cast<uint4*>(&ws[(s * 32 + n) * WS + v * 8]) =
+                *reinterpret_cast<const uint4*>(wsrc + ((size_t)s * a.np + n) * 32 + v * 8);
+        }
+        __syncthreads();
+#pragma unroll 3
+        for (int s = 0; s < KS; ++s) {
+            const int kf = s * 32 + g * 8;
+            const int tap = kf / CC, ci0 = kf - tap * CC;
+            bf16x8 af[2], bfr[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi) {
+                const int row = wave * 2 + mi;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (tap < 9) {
+                    const int dy = tap / 3, dx = tap - dy * 3;
+                    v = *reinterpret_cast<const uint4*>(&xs[((row + dy) * 18 + r16 + dx) * XS + ci0]);
+                }
+                af[mi] = as_bf16x8(v);
+            }
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni)
+                bfr[ni] = as_bf16x8(*reinterpret_cast<const uint4*>(&ws[(s * 32 + ni * 16 + r16) * WS + g * 8]));
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bfr[ni], acc[mi][ni], 0, 0, 0);
+        }
+    }
+
+    // D layout of a 16x16 tile: lane (g, r16), register i -> pixel x = 4g + i, channel r16
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi) {
+        const int y = y0 + wave * 2 + mi;
+        if (y >= a.H) continue;
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+            const int n = nc * 32 + ni * 16 + r16;
+            const float bias = (a.bias && n < a.nb) ? a.bias[n] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = x0 + g * 4 + i;
+                if (x >= a.W) continue;
+                float v = acc[mi][ni][i] + bias;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                const size_t pix = (size_t)(b * a.H + y) * a.W + x;
+                if (a.outf) a.outf[pix * a.np + n] = v;
+                else if (n < a.oc0) a.out0[pix * a.oc0 + n] = f2bf(v);
+                else a.out1[pix * a.oc1 + (n - a.oc0)] = f2bf(v);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------- weight gradient (split-K)
+struct WgradArgs {
+    const uint16_t* in0;
+    const uint16_t* in1;
+    int c0, c1;
+    const uint16_t* g;     // masked output gradient, NHWC with np channels
+    int np;
+    float* part;           // [nsplit][cin/CC][KS][np][32]
+    float* bpart;          // [nsplit][np]
+    int B, H, W, nsplit, tiles_x, tiles_y;
+};
+
+template <int CC>
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs a) {
+    constexpr int KS = (9 * CC + 31) / 32;
+    constexpr int XS = CC + 8;
+    constexpr int VPP = CC / 8;
+    constexpr int GSP = 136;   // transposed gradient tile row stride (128 pixels + pad)
+    __shared__ __align__(16) uint16_t xs[180 * XS];
+    __shared__ __align__(16) uint16_t gs[32 * GSP];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int mi = wave >> 1, ni = wave & 1;   // this wave's 16x16 tile inside each 32x32 K step
+    const int split = blockIdx.x, nc = blockIdx.y, c = blockIdx.z;
+    const int nchunk = (a.c0 + a.c1) / CC;
+    const int ntiles = a.B * a.tiles_y * a.tiles_x;
+    const int per = (ntiles + a.nsplit - 1) / a.nsplit;
+    const int t0 = split * per;
+    const int t1 = min(ntiles, t0 + per);
+
+    const int cb = c * CC;
+    const uint16_t* src;
+    int cs, co;
+    if (cb < a.c0) { src = a.in0; cs = a.c0; co = cb; }
+    else { src = a.in1; cs = a.c1; co = cb - a.c0; }
+
+    f32x4 acc[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bacc = 0.0f;
+
+    for (int t = t0; t < t1; ++t) {
+        const int b = t / (a.tiles_y * a.tiles_x);
+        const int r = t - b * a.tiles_y * a.tiles_x;
+        const int ty = r / a.tiles_x, tx = r - ty * a.tiles_x;
+        const int x0 = tx * 16, y0 = ty * 8;
+        __syncthreads();
+        for (int i = tid; i < 180 * VPP; i += 256) {
+            const int p = i / VPP, v = i - p * VPP;
+            const int hy = p / 18, hx = p - hy * 18;
+            const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                val = *reinterpret_cast<const uint4*>(src + ((size_t)(b * a.H + gy) * a.W + gx) * cs + co + v * 8);
+            *reinterpret_cast<uint4*>(&xs[p * XS + v * 8]) = val;
+        }
+        for (int i = tid; i < 128 * 4; i += 256) {
+            const int p = i >> 2, v = i & 3;
+            const int gy = y0 + (p >> 4), gx = x0 + (p & 15);
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (gy < a.H && gx < a.W)
+                val = *reinterpret_cast<const uint4*>(a.g + ((size_t)(b * a.H + gy) * a.W + gx) * a.np + nc * 32 + v * 8);
+            const uint16_t* e = reinterpret_cast<const uint16_t*>(&val);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) gs[(v * 8 + k) * GSP + p] = e[k];
+        }
+        __syncthreads();
+        if (c == 0 && tid < 32) {
+            float sacc = 0.0f;
+            for (int p = 0; p < 128; ++p) sacc += bf2f(gs[tid * GSP + p]);
+            bacc += sacc;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {   // 32 pixels per K step: rows 2q, 2q+1
+            const int row = 2 * q + (g >> 1), xb = 8 * (g & 1);
+            const bf16x8 bfr = as_bf16x8(*reinterpret_cast<const uint4*>(&gs[(ni * 16 + r16) * GSP + row * 16 + xb]));
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                const int kf = s * 32 + mi * 16 + r16;
+                const int tap = kf / CC, ci = kf - tap * CC;
+                uint16_t e[8];
+                if (tap < 9) {
+                    const int dy = tap / 3, dx = tap - dy * 3;
+                    const uint16_t* base = &xs[((row + dy) * 18 + xb + dx) * XS + ci];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) e[j] = base[j * XS];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) e[j] = 0;
+                }
+                uint4 v;
+                v.x = (uint32_t)e[0] | ((uint32_t)e[1] << 16);
+                v.y = (uint32_t)e[2] | ((uint32_t)e[3] << 16);
+                v.z = (uint32_t)e[4] | ((uint32_t)e[5] << 16);
+                v.w = (uint32_t)e[6] | ((uint32_t)e[7] << 16);
+                acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(v), bfr, acc[s], 0, 0, 0);
+            }
+        }
+    }
+    // tile (s, mi, ni): lane (g, r16), register i -> K row mi*16 + 4g + i, channel ni*16 + r16
+    float* P = a.part + ((size_t)split * nchunk + c) * KS * a.np * 32;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        float4 v = make_float4(acc[s][0], acc[s][1], acc[s][2], acc[s][3]);
+        *reinterpret_cast<float4*>(&P[((size_t)s * a.np + nc * 32 + ni * 16 + r16) * 32 + mi * 16 + 4 * g]) = v;
+    }
+    if (c == 0 && tid < 32) a.bpart[(size_t)split * a.np + nc * 32 + tid] = bacc;
+}
+
+// Sums the split partials in a fixed order into the logical layout
+// W[tap][cin][cout] followed by bias[cout].
+__global__ __launch_bounds__(256) void k_wgrad_reduce(const float* __restrict__ part, const float* __restrict__ bpart,
+                                                      float* __restrict__ gw, int nsplit, int nchunk, int KS, int CC,
+                                                      int np, int cin, int cout) {
+    const int nw = 9 * cin * cout;
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= nw + cout) return;
+    float s = 0.0f;
+    if (idx < nw) {
+        const int tap = idx / (cin * cout);
+        const int rem = idx - tap * cin * cout;
+        const int ci = rem / cout, n = rem - (rem / cout) * cout;
+        const int c = ci / CC, kf = tap * CC + (ci - c * CC);
+        const int st = kf >> 5, j = kf & 31;
+        const size_t stride = (size_t)nchunk * KS * np * 32;
+        const size_t off = (((size_t)c * KS + st) * np + n) * 32 + j;
+        for (int p = 0; p < nsplit; ++p) s += part[p * stride + off];
+    } else {
+        const int n = idx - nw;
+        for (int p = 0; p < nsplit; ++p) s += bpart[(size_t)p * np + n];
+    }
+    gw[idx] = s;
+}
+
+// ------------------------------------------------------------- weight packing
+// Forward packing: [c][s][n][j] = W[tap][ci][n], K index s*32+j = tap*CC + (ci mod CC).
+__global__ __launch_bounds__(256) void k_pack_fwd(const float* __restrict__ w, uint16_t* __restrict__ wf, int cin,
+                                                  int cout, int np, int CC, int KS, int total) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int j = idx & 31;
+    const int n = (idx >> 5) % np;
+    const int s = (idx / (32 * np)) % KS;
+    const int c = idx / (32 * np * KS);
+    const int kf = s * 32 + j;
+    const int tap = kf / CC, ci = c * CC + (kf - tap * CC);
+    float v = 0.0f;
+    if (tap < 9 && n < cout && ci < cin) v = w[((size_t)tap * cin + ci) * cout + n];
+    wf[idx] = f2bf(v);
+}
+
+// Input-gradient packing: a forward convolution over the output gradient with
+// W'[t][co][ci] = W[8-t][ci][co]; chunks of 32 over cout (padded to np).
+__global__ __launch_bounds__(256) void k_pack_dgrad(const float* __restrict__ w, uint16_t* __restrict__ wd, int cin,
+                                                    int cout, int total) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int j = idx & 31;
+    const int n = (idx >> 5) % cin;
+    const int s = (idx / (32 * cin)) % 9;
+    const int c = idx / (32 * cin * 9);
+    const int co = c * 32 + j;
+    float v = 0.0f;
+    if (co < cout) v = w[((size_t)(8 - s) * cin + n) * cout + co];
+    wd[idx] = f2bf(v);
+}
+
+// ------------------------------------------------------- elementwise kernels
+__global__ __launch_bounds__(256) void k_relu_mask(uint16_t* __restrict__ g, const uint16_t* __restrict__ y,
+                                                   size_t n8) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n8) return;
+    uint4 gv = reinterpret_cast<uint4*>(g)[i];
+    const uint4 yv = reinterpret_cast<const uint4*>(y)[i];
+    uint16_t* ge = reinterpret_cast<uint16_t*>(&gv);
+    const uint16_t* ye = reinterpret_cast<const uint16_t*>(&yv);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (!(bf2f(ye[k]) > 0.0f)) ge[k] = 0;
+    reinterpret_cast<uint4*>(g)[i] = gv;
+}
+
+// 2x2 max-pool, H and W even.
+__global__ __launch_bounds__(256) void k_pool_fwd(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int B,
+                                                  int H, int W, int C) {
+    const int h = H >> 1, w = W >> 1, cv = C >> 3;
+    const size_t total = (size_t)B * h * w * cv;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const int v = (int)(i % cv);
+    size_t r = i / cv;
+    const int x = (int)(r % w);
+    r /= w;
+    const int y = (int)(r % h);
+    const int b = (int)(r / h);
+    uint4 q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        q[k] = *reinterpret_cast<const uint4*>(in + ((size_t)(b * H + 2 * y + (k >> 1)) * W + 2 * x + (k & 1)) * C + v * 8);
+    uint4 o;
+    uint16_t* oe = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        uint16_t best = reinterpret_cast<const uint16_t*>(&q[0])[e];
+        float bv = bf2f(best);
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            const uint16_t cand = reinterpret_cast<const uint16_t*>(&q[k])[e];
+            if (bf2f(cand) > bv) { bv = bf2f(cand); best = cand; }
+        }
+        oe[e] = best;
+    }
+    *reinterpret_cast<uint4*>(out + ((size_t)(b * h + y) * w + x) * C + v * 8) = o;
+}
+
+// Routes the pooled gradient to the first maximum of each 2x2 window (scan
+// order (0,0),(0,1),(1,0),(1,1)); adds into gin when accumulate != 0.
+__global__ __launch_bounds__(256) void k_pool_bwd(const uint16_t* __restrict__ in, const uint16_t* __restrict__ gout,
+                                                  uint16_t* __restrict__ gin, int accumulate, int B, int H, int W,
+                                                  int C) {
+    const int h = H >> 1, w = W >> 1, cv = C >> 3;
+    const size_t total = (size_t)B * h * w * cv;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const int v = (int)(i % cv);
+    size_t r = i / cv;
+    const int x = (int)(r % w);
+    r /= w;
+    const int y = (int)(r % h);
+    const int b = (int)(r / h);
+    uint4 q[4], gi[4];
+    size_t off[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        off[k] = ((size_t)(b * H + 2 * y + (k >> 1)) * W + 2 * x + (k & 1)) * C + v * 8;
+        q[k] = *reinterpret_cast<const uint4*>(in + off[k]);
+        gi[k] = accumulate ? *reinterpret_cast<const uint4*>(gin + off[k]) : make_uint4(0u, 0u, 0u, 0u);
+    }
+    const uint4 go = *reinterpret_cast<const uint4*>(gout + ((size_t)(b * h + y) * w + x) * C + v * 8);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        int arg = 0;
+        float bv = bf2f(reinterpret_cast<const uint16_t*>(&q[0])[e]);
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            const float cv2 = bf2f(reinterpret_cast<const uint16_t*>(&q[k])[e]);
+            if (cv2 > bv) { bv = cv2; arg = k; }
+        }
+        uint16_t* t = reinterpret_cast<uint16_t*>(&gi[arg]);
+        t[e] = f2bf(bf2f(t[e]) + bf2f(reinterpret_cast<const uint16_t*>(&go)[e]));
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) *reinterpret_cast<uint4*>(gin + off[k]) = gi[k];
+}
+
+// Bilinear source taps of output coordinate X for a x2 upsample of n samples
+// (half-pixel centres, clamped at the low edge and the high edge).
+__device__ inline void up_taps(int X, int n, int& i0, int& i1, float& l1) {
+    const float s = fmaxf(0.5f * ((float)X + 0.5f) - 0.5f, 0.0f);
+    i0 = (int)s;
+    i1 = i0 + (i0 < n - 1 ? 1 : 0);
+    l1 = s - (float)i0;
+}
+
+__global__ __launch_bounds__(256) void k_up_fwd(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int B,
+                                                int h, int w, int C) {
+    const int H = 2 * h, W = 2 * w, cv = C >> 3;
+    const size_t total = (size_t)B * H * W * cv;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const int v = (int)(i % cv);
+    size_t r = i / cv;
+    const int X = (int)(r % W);
+    r /= W;
+    const int Y = (int)(r % H);
+    const int b = (int)(r / H);
+    int y0, y1, x0, x1;
+    float ly, lx;
+    up_taps(Y, h, y0, y1, ly);
+    up_taps(X, w, x0, x1, lx);
+    const uint16_t* base = in + (size_t)b * h * w * C + v * 8;
+    const uint4 q00 = *reinterpret_cast<const uint4*>(base + ((size_t)y0 * w + x0) * C);
+    const uint4 q01 = *reinterpret_cast<const uint4*>(base + ((size_t)y0 * w + x1) * C);
+    const uint4 q10 = *reinterpret_cast<const uint4*>(base + ((size_t)y1 * w + x0) * C);
+    const uint4 q11 = *reinterpret_cast<const uint4*>(base + ((size_t)y1 * w + x1) * C);
+    uint4 o;
+    uint16_t* oe = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float a00 = bf2f(reinterpret_cast<const uint16_t*>(&q00)[e]);
+        const float a01 = bf2f(reinterpret_cast<const uint16_t*>(&q01)[e]);
+        const float a10 = bf2f(reinterpret_cast<const uint16_t*>(&q10)[e]);
+        const float a11 = bf2f(reinterpret_cast<const uint16_t*>(&q11)[e]);
+        const float top = (1.0f - lx) * a00 + lx * a01;
+        const float bot = (1.0f - lx) * a10 + lx * a11;
+        oe[e] = f2bf((1.0f - ly) * top + ly * bot);
+    }
+    *reinterpret_cast<uint4*>(out + ((size_t)(b * H + Y) * W + X) * C + v * 8) = o;
+}
+
+// Weight with which input sample i contributes to output coordinate X.
+__device__ inline float up_weight(int X, int n, int i) {
+    int i0, i1;
+    float l1;
+    up_taps(X, n, i0, i1, l1);
+    return (i0 == i ? 1.0f - l1 : 0.0f) + (i1 == i ? l1 : 0.0f);
+}
+
+// Transpose of k_up_fwd in gather form (each input sample sums the output
+// samples that read it, fixed order).
+__global__ __launch_bounds__(256) void k_up_bwd(const uint16_t* __restrict__ gout, uint16_t* __restrict__ gin, int B,
+                                                int h, int w, int C) {
+    const int H = 2 * h, W = 2 * w, cv = C >> 3;
+    const size_t total = (size_t)B * h * w * cv;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const int v = (int)(i % cv);
+    size_t r = i / cv;
+    const int x = (int)(r % w);
+    r /= w;
+    const int y = (int)(r % h);
+    const int b = (int)(r / h);
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int Y = max(0, 2 * y - 2); Y <= min(H - 1, 2 * y + 2); ++Y) {
+        const float wy = up_weight(Y, h, y);
+        if (wy == 0.0f) continue;
+        for (int X = max(0, 2 * x - 2); X <= min(W - 1, 2 * x + 2); ++X) {
+            const float wx = up_weight(X, w, x);
+            if (wx == 0.0f) continue;
+            const uint4 q = *reinterpret_cast<const uint4*>(gout + ((size_t)(b * H + Y) * W + X) * C + v * 8);
+            const float wgt = wy * wx;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc[e] += wgt * bf2f(reinterpret_cast<const uint16_t*>(&q)[e]);
+        }
+    }
+    uint4 o;
+    uint16_t* oe = reinterpret_cast<uint16_t*>(&o);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) oe[e] = f2bf(acc[e]);
+    *reinterpret_cast<uint4*>(gin + ((size_t)(b * h + y) * w + x) * C + v * 8) = o;
+}
+
+// Guide images -> 16-channel bf16 input at padded size (edge replication):
+// log1p radiance (3), albedo (3), normal (3), log1p depth (1), zeros (6).
+__global__ __launch_bounds__(256) void k_features(const float4* __restrict__ rad, const float4* __restrict__ alb,
+                                                  const float4* __restrict__ nrm, const float* __restrict__ dep,
+                                                  uint16_t* __restrict__ x0, int B, int H, int W, int Hp, int Wp) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)B * Hp * Wp;
+    if (i >= total) return;
+    const int x = (int)(i % Wp);
+    const int y = (int)((i / Wp) % Hp);
+    const int b = (int)(i / ((size_t)Wp * Hp));
+    const size_t s = ((size_t)b * H + min(y, H - 1)) * W + min(x, W - 1);
+    float f[16];
+    const float4 r = rad[s];
+    f[0] = log1pf(fmaxf(r.x, 0.0f));
+    f[1] = log1pf(fmaxf(r.y, 0.0f));
+    f[2] = log1pf(fmaxf(r.z, 0.0f));
+    const float4 a = alb ? alb[s] : make_float4(0, 0, 0, 0);
+    const float4 n = nrm ? nrm[s] : make_float4(0, 0, 0, 0);
+    f[3] = a.x; f[4] = a.y; f[5] = a.z;
+    f[6] = n.x; f[7] = n.y; f[8] = n.z;
+    f[9] = dep ? log1pf(fmaxf(dep[s], 0.0f)) : 0.0f;
+#pragma unroll
+    for (int k = 10; k < 16; ++k) f[k] = 0.0f;
+    uint4 o[2];
+    uint16_t* oe = reinterpret_cast<uint16_t*>(o);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) oe[k] = f2bf(f[k]);
+    reinterpret_cast<uint4*>(x0)[i * 2] = o[0];
+    reinterpret_cast<uint4*>(x0)[i * 2 + 1] = o[1];
+}
+
+// Tonemap of the training loss: T(x) = (ACES(0.5 x) + 1e-3)^(1/2.2), ACES
+// clamped to [0, 1]; returns T and dT/dx.
+__device__ inline float tonemap_loss(float x, float& dT) {
+    const float u = 0.5f * fmaxf(x, 0.0f);
+    const float num = u * (2.51f * u + 0.03f);
+    const float den = u * (2.43f * u + 0.59f) + 0.14f;
+    float f = num / den;
+    float df = 0.0f;
+    if (x > 0.0f && f < 1.0f) {
+        const float dnum = 5.02f * u + 0.03f, dden = 4.86f * u + 0.59f;
+        df = 0.5f * (dnum * den - num * dden) / (den * den);
+    }
+    f = fminf(fmaxf(f, 0.0f), 1.0f);
+    const float p = powf(f + 1e-3f, 1.0f / 2.2f);
+    dT = df * (1.0f / 2.2f) * p / (f + 1e-3f);
+    return p;
+}
+
+struct KpnArgs {
+    const float* logits;   // [B][Hp][Wp][32]
+    const float4* rad;     // [B][H][W]
+    const float4* target;  // [B][H][W] or null
+    float4* out;           // [B][H][W] or null
+    uint16_t* dlogits;     // [B][Hp][Wp][32] or null
+    float* loss_part;      // [gridDim.x] or null
+    int B, H, W, Hp, Wp;
+    float inv_count;       // 1 / (B*H*W*3)
+};
+
+// Softmax over the 25 logits, filter of the noisy radiance (edge-clamped
+// 5x5 window), and for training the L1 tonemapped loss with its gradient.
+__global__ __launch_bounds__(256) void k_kpn(KpnArgs a) {
+    __shared__ float red[256];
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)a.B * a.Hp * a.Wp;
+    float lsum = 0.0f;
+    if (i < total) {
+        const int x = (int)(i % a.Wp);
+        const int y = (int)((i / a.Wp) % a.Hp);
+        const int b = (int)(i / ((size_t)a.Wp * a.Hp));
+        if (y < a.H && x < a.W) {
+            const float* lg = a.logits + i * 32;
+            float s[25];
+            float mx = lg[0];
+#pragma unroll
+            for (int k = 1; k < 25; ++k) mx = fmaxf(mx, lg[k]);
+            float z = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 25; ++k) { s[k] = expf(lg[k] - mx); z += s[k]; }
+            const float iz = 1.0f / z;
+            float3 o = make_float3(0, 0, 0);
+            float3 rk[25];
+#pragma unroll
+            for (int k = 0; k < 25; ++k) {
+                s[k] *= iz;
+                const int yy = min(max(y + k / 5 - 2, 0), a.H - 1), xx = min(max(x + k % 5 - 2, 0), a.W - 1);
+                const float4 r = a.rad[((size_t)b * a.H + yy) * a.W + xx];
+                rk[k] = make_float3(r.x, r.y, r.z);
+                o.x += s[k] * r.x; o.y += s[k] * r.y; o.z += s[k] * r.z;
+            }
+            const size_t pix = ((size_t)b * a.H + y) * a.W + x;
+            if (a.out) a.out[pix] = make_float4(o.x, o.y, o.z, 1.0f);
+            if (a.target) {
+                const float4 t = a.target[pix];
+                float d0, d1, d2, dd;
+                const float e0 = tonemap_loss(o.x, d0) - tonemap_loss(t.x, dd);
+                const float e1 = tonemap_loss(o.y, d1) - tonemap_loss(t.y, dd);
+                const float e2 = tonemap_loss(o.z, d2) - tonemap_loss(t.z, dd);
+                lsum = fabsf(e0) + fabsf(e1) + fabsf(e2);
+                const float g0 = (e0 > 0.0f ? 1.0f : (e0 < 0.0f ? -1.0f : 0.0f)) * d0 * a.inv_count;
+                const float g1 = (e1 > 0.0f ? 1.0f : (e1 < 0.0f ? -1.0f : 0.0f)) * d1 * a.inv_count;
+                const float g2 = (e2 > 0.0f ? 1.0f : (e2 < 0.0f ? -1.0f : 0.0f)) * d2 * a.inv_count;
+                if (a.dlogits) {
+                    uint16_t dl[32];
+#pragma unroll
+                    for (int k = 0; k < 25; ++k)
+                        dl[k] = f2bf(s[k] * (g0 * (rk[k].x - o.x) + g1 * (rk[k].y - o.y) + g2 * (rk[k].z - o.z)));
+#pragma unroll
+                    for (int k = 25; k < 32; ++k) dl[k] = 0;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        reinterpret_cast<uint4*>(a.dlogits + i * 32)[v] = reinterpret_cast<const uint4*>(dl)[v];
+                }
+            }
+        } else if (a.dlogits) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) reinterpret_cast<uint4*>(a.dlogits + i * 32)[v] = make_uint4(0u, 0u, 0u, 0u);
+        }
+    }
+    if (a.loss_part) {
+        red[threadIdx.x] = lsum;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) a.loss_part[blockIdx.x] = red[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sum_serial(const float* __restrict__ in, int n, float scale,
+                                                    float* __restrict__ out) {
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < n; i += 256) s += in[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, size_t n, float lr,
+                                              float b1, float b2, float eps, float bc1, float bc2) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= (lr / bc1) * mi / (sqrtf(vi) / sqrtf(bc2) + eps);
+}
+
+__global__ __launch_bounds__(256) void k_bf16_to_f32(const uint16_t* __restrict__ in, float* __restrict__ out,
+                                                     size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = bf2f(in[i]);
+}
+
+// ------------------------------------------------------------- network tables
+enum : int { X0, A0, E0, P1, A1, E1, P2, A2, E2, P3, A3, E3, P4, A4, E4,
+             U3, B3, D3, U2, B2, D2, U1, B1, D1, U0, B0, D0, kNumAct };
+constexpr int kLogits = kNumAct;   // tensor index of the head logits
+constexpr int kActLevel[kNumAct] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4,
+                                    3, 3, 3, 2, 2, 2, 1, 1, 1, 0, 0, 0};
+constexpr int kActChan[kNumAct] = {16, 32, 32, 32, 64, 64, 64, 96, 96, 96, 128, 128, 128, 160, 160,
+                                   160, 128, 128, 128, 96, 96, 96, 64, 64, 64, 32, 32};
+struct LayerSpec { int in0, in1, out, cout, relu; };
+constexpr int kNumLayers = 19;
+constexpr LayerSpec kLayers[kNumLayers] = {
+    {X0, -1, A0, 32, 1},  {A0, -1, E0, 32, 1},   {P1, -1, A1, 64, 1},   {A1, -1, E1, 64, 1},
+    {P2, -1, A2, 96, 1},  {A2, -1, E2, 96, 1},   {P3, -1, A3, 128, 1},  {A3, -1, E3, 128, 1},
+    {P4, -1, A4, 160, 1}, {A4, -1, E4, 160, 1},  {U3, E3, B3, 128, 1},  {B3, -1, D3, 128, 1},
+    {U2, E2, B2, 96, 1},  {B2, -1, D2, 96, 1},   {U1, E1, B1, 64, 1},   {B1, -1, D1, 64, 1},
+    {U0, E0, B0, 32, 1},  {B0, -1, D0, 32, 1},   {D0, -1, kLogits, 25, 0}};
+constexpr int kInputChannels = 10;   // real channels of X0 (the rest are zero padding)
+
+struct Layer {
+    LayerSpec sp;
+    int cin0, cin1, cin, cout, np, cc, ks, nchunk;
+    size_t w_off, b_off;   // offsets into the FP32 parameter vector
+    DevBuf wf, wd;         // packed bf16 weights: forward, input gradient
+};
+
+}  // namespace
+
+struct ohpt_denoiser {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Layer> layers;
+    size_t nparam = 0;
+    DevBuf params, grads, adam_m, adam_v;
+    int adam_t = 0;
+    bool have_grads = false;
+    bool profile = false;
+
+    size_t cap_px = 0;     // level-0 pixel capacity (batch * Hp * Wp)
+    int B = 0, H = 0, W = 0, Hp = 0, Wp = 0;   // shape of the last call
+    DevBuf act[kNumAct], gact[kNumAct];
+    DevBuf logits, dlogits;
+    DevBuf wpart, bpart, loss_part, loss_dev;
+    DevBuf in_rad, in_alb, in_nrm, in_dep, in_tgt, out_stage;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> conv_ev;
+    int conv_ev_used = 0;
+    ohpt_denoiser_stats stats{};
+};
+
+namespace {
+
+int pad16(int v) { return (v + 15) & ~15; }
+
+hipError_t ensure_capacity(ohpt_denoiser* d, int B, int Hp, int Wp) {
+    const size_t px = (size_t)B * Hp * Wp;
+    if (px <= d->cap_px) return hipSuccess;
+    for (int t = 0; t < kNumAct; ++t) {
+        const size_t n = (px >> (2 * kActLevel[t])) * kActChan[t] * 2;
+        hipError_t e = d->act[t].ensure(n);
+        if (e != hipSuccess) return e;
+        if (t != X0) {
+            e = d->gact[t].ensure(n);
+            if (e != hipSuccess) return e;
+        }
+    }
+    hipError_t e = d->logits.ensure(px * 32 * 4);
+    if (e == hipSuccess) e = d->dlogits.ensure(px * 32 * 2);
+    if (e == hipSuccess) e = d->loss_part.ensure((px / 256 + 1) * 4);
+    if (e == hipSuccess) d->cap_px = px;
+    return e;
+}
+
+void launch_conv(ohpt_denoiser* d, const Layer& L, const ConvArgs& a, int cc) {
+    dim3 grid(cdiv(a.W, 16), cdiv(a.H, 8), (uint32_t)(a.B * (a.np / 32)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (d->profile) {
+        if ((int)d->conv_ev.size() < d->conv_ev_used + 2) {
+            hipEvent_t x, y;
+            (void)hipEventCreate(&x);
+            (void)hipEventCreate(&y);
+            d->conv_ev.push_back(x);
+            d->conv_ev.push_back(y);
+        }
+        e0 = d->conv_ev[d->conv_ev_used];
+        e1 = d->conv_ev[d->conv_ev_used + 1];
+        d->conv_ev_used += 2;
+        (void)hipEventRecord(e0, d->stream);
+    }
+    if (cc == 16) k_conv3x3<16><<<grid, 256, 0, d->stream>>>(a);
+    else k_conv3x3<32><<<grid, 256, 0, d->stream>>>(a);
+    if (d->profile) (void)hipEventRecord(e1, d->stream);
+    (void)L;
+}
+
+void conv_forward(ohpt_denoiser* d, const Layer& L, int lvl) {
+    const int H = d->Hp >> lvl, W = d->Wp >> lvl;
+    ConvArgs a{};
+    a.in0 = d->act[L.sp.in0].as<uint16_t>();
+    a.in1 = L.sp.in1 >= 0 ? d->act[L.sp.in1].as<uint16_t>() : nullptr;
+    a.c0 = L.cin0;
+    a.c1 = L.cin1;
+    a.wp = L.wf.as<uint16_t>();
+    a.np = L.np;
+    a.bias = d->params.as<float>() + L.b_off;
+    a.nb = L.cout;
+    if (L.sp.out == kLogits) {
+        a.outf = d->logits.as<float>();
+    } else {
+        a.out0 = d->act[L.sp.out].as<uint16_t>();
+        a.oc0 = L.np;
+    }
+    a.relu = L.sp.relu;
+    a.B = d->B;
+    a.H = H;
+    a.W = W;
+    launch_conv(d, L, a, L.cc);
+}
+
+// Input gradient of layer L from G (its masked output gradient).
+void conv_dgrad(ohpt_denoiser* d, const Layer& L, const uint16_t* G, int lvl) {
+    const int H = d->Hp >> lvl, W = d->Wp >> lvl;
+    ConvArgs a{};
+    a.in0 = G;
+    a.c0 = L.np;
+    a.c1 = 0;
+    a.wp = L.wd.as<uint16_t>();
+    a.np = L.cin;
+    a.out0 = d->gact[L.sp.in0].as<uint16_t>();
+    a.oc0 = L.cin0;
+    a.out1 = L.sp.in1 >= 0 ? d->gact[L.sp.in1].as<uint16_t>() : nullptr;
+    a.oc1 = L.cin1;
+    a.B = d->B;
+    a.H = H;
+    a.W = W;
+    launch_conv(d, L, a, 32);
+}
+
+hipError_t conv_wgrad(ohpt_denoiser* d, const Layer& L, const uint16_t* G, int lvl) {
+    const int H = d->Hp >> lvl, W = d->Wp >> lvl;
+    const int tx = (int)cdiv(W, 16), ty = (int)cdiv(H, 8);
+    const int ntiles = d->B * tx * ty;
+    const int combos = (L.np / 32) * L.nchunk;
+    int nsplit = std::max(1, 2048 / combos);
+    nsplit = std::min(nsplit, ntiles);
+    const size_t part_n = (size_t)nsplit * L.nchunk * L.ks * L.np * 32;
+    hipError_t e = d->wpart.ensure(part_n * 4);
+    if (e == hipSuccess) e = d->bpart.ensure((size_t)nsplit * L.np * 4);
+    if (e != hipSuccess) return e;
+    WgradArgs a{};
+    a.in0 = d->act[L.sp.in0].as<uint16_t>();
+    a.in1 = L.sp.in1 >= 0 ? d->act[L.sp.in1].as<uint16_t>() : nullptr;
+    a.c0 = L.cin0;
+    a.c1 = L.cin1;
+    a.g = G;
+    a.np = L.np;
+    a.part = d->wpart.as<float>();
+    a.bpart = d->bpart.as<float>();
+    a.B = d->B;
+    a.H = H;
+    a.W = W;
+    a.nsplit = nsplit;
+    a.tiles_x = tx;
+    a.tiles_y = ty;
+    dim3 grid(nsplit, L.np / 32, L.nchunk);
+    if (L.cc == 16) k_wgrad<16><<<grid, 256, 0, d->stream>>>(a);
+    else k_wgrad<32><<<grid, 256, 0, d->stream>>>(a);
+    const int total = 9 * L.cin * L.cout + L.cout;
+    k_wgrad_reduce<<<cdiv(total, 256), 256, 0, d->stream>>>(a.part, a.bpart, d->grads.as<float>() + L.w_off,
+                                                            nsplit, L.nchunk, L.ks, L.cc, L.np, L.cin, L.cout);
+    return hipSuccess;
+}
+
+void relu_mask(ohpt_denoiser* d, int t) {
+    const size_t n = ((size_t)d->B * d->Hp * d->Wp >> (2 * kActLevel[t])) * kActChan[t];
+    k_relu_mask<<<cdiv(n / 8, 256), 256, 0, d->stream>>>(d->gact[t].as<uint16_t>(), d->act[t].as<uint16_t>(), n / 8);
+}
+
+void pool_forward(ohpt_denoiser* d, int src, int dst) {
+    const int lvl = kActLevel[src], C = kActChan[src];
+    const int H = d->Hp >> lvl, W = d->Wp >> lvl;
+    const size_t n = (size_t)d->B * (H / 2) * (W / 2) * (C / 8);
+    k_pool_fwd<<<cdiv(n, 256), 256, 0, d->stream>>>(d->act[src].as<uint16_t>(), d->act[dst].as<uint16_t>(), d->B, H,
+                                                    W, C);
+}
+
+void pool_backward(ohpt_denoiser* d, int src, int dst) {   // dst = pooled tensor, gradient flows into src
+    const int lvl = kActLevel[src], C = kActChan[src];
+    const int H = d->Hp >> lvl, W = d->Wp >> lvl;
+    const size_t n = (size_t)d->B * (H / 2) * (W / 2) * (C / 8);
+    k_pool_bwd<<<cdiv(n, 256), 256, 0, d->stream>>>(d->act[src].as<uint16_t>(), d->gact[dst].as<uint16_t>(),
+                                                    d->gact[src].as<uint16_t>(), 1, d->B, H, W, C);
+}
+
+void up_forward(ohpt_denoiser* d, int src, int dst) {
+    const int lvl = kActLevel[src], C = kActChan[src];
+    const int h = d->Hp >> lvl, w = d->Wp >> lvl;
+    const size_t n = (size_t)d->B * (2 * h) * (2 * w) * (C / 8);
+    k_up_fwd<<<cdiv(n, 256), 256, 0, d->stream>>>(d->act[src].as<uint16_t>(), d->act[dst].as<uint16_t>(), d->B, h, w,
+                                                  C);
+}
+
+void up_backward(ohpt_denoiser* d, int src, int dst) {   // gradient of dst (upsampled) -> src
+    const int lvl = kActLevel[src], C = kActChan[src];
+    const int h = d->Hp >> lvl, w = d->Wp >> lvl;
+    const size_t n = (size_t)d->B * h * w * (C / 8);
+    k_up_bwd<<<cdiv(n, 256), 256, 0, d->stream>>>(d->gact[dst].as<uint16_t>(), d->gact[src].as<uint16_t>(), d->B, h,
+                                                  w, C);
+}
+
+double conv_flops(const ohpt_denoiser* d, int H, int W, bool training) {
+    double f = 0.0;
+    for (const Layer& L : d->layers) {
+        const int lvl = kActLevel[L.sp.in0];
+        const double px = (double)d->B * (double)(H >> lvl) * (double)(W >> lvl);
+        const double cin = (L.sp.in0 == X0) ? kInputChannels : (double)L.cin;
+        const double fwd = 2.0 * 9.0 * cin * L.cout * px;
+        f += fwd;
+        if (training) f += fwd + (L.sp.in0 == X0 ? 0.0 : fwd);   // weight gradient + input gradient
+    }
+    return f;
+}
+
+void network_forward(ohpt_denoiser* d) {
+    const auto& Ls = d->layers;
+    conv_forward(d, Ls[0], 0);
+    conv_forward(d, Ls[1], 0);
+    pool_forward(d, E0, P1);
+    conv_forward(d, Ls[2], 1);
+    conv_forward(d, Ls[3], 1);
+    pool_forward(d, E1, P2);
+    conv_forward(d, Ls[4], 2);
+    conv_forward(d, Ls[5], 2);
+    pool_forward(d, E2, P3);
+    conv_forward(d, Ls[6], 3);
+    conv_forward(d, Ls[7], 3);
+    pool_forward(d, E3, P4);
+    conv_forward(d, Ls[8], 4);
+    conv_forward(d, Ls[9], 4);
+    up_forward(d, E4, U3);
+    conv_forward(d, Ls[10], 3);
+    conv_forward(d, Ls[11], 3);
+    up_forward(d, D3, U2);
+    conv_forward(d, Ls[12], 2);
+    conv_forward(d, Ls[13], 2);
+    up_forward(d, D2, U1);
+    conv_forward(d, Ls[14], 1);
+    conv_forward(d, Ls[15], 1);
+    up_forward(d, D1, U0);
+    conv_forward(d, Ls[16], 0);
+    conv_forward(d, Ls[17], 0);
+    conv_forward(d, Ls[18], 0);
+}
+
+// Backward of one ReLU convolution given the gradient of its output tensor.
+hipError_t layer_backward(ohpt_denoiser* d, int li, bool need_input_grad) {
+    const Layer& L = d->layers[li];
+    const int lvl = kActLevel[L.sp.in0];
+    const uint16_t* G;
+    if (L.sp.out == kLogits) {
+        G = d->dlogits.as<uint16_t>();
+    } else {
+        relu_mask(d, L.sp.out);
+        G = d->gact[L.sp.out].as<uint16_t>();
+    }
+    hipError_t e = conv_wgrad(d, L, G, lvl);
+    if (e != hipSuccess) return e;
+    if (need_input_grad) conv_dgrad(d, L, G, lvl);
+    return hipSuccess;
+}
+
+hipError_t network_backward(ohpt_denoiser* d) {
+    hipError_t e;
+#define BW(i) do { e = layer_backward(d, (i), true); if (e != hipSuccess) return e; } while (0)
+    BW(18);
+    BW(17);
+    BW(16);
+    up_backward(d, D1, U0);
+    BW(15);
+    BW(14);
+    up_backward(d, D2, U1);
+    BW(13);
+    BW(12);
+    up_backward(d, D3, U2);
+    BW(11);
+    BW(10);
+    up_backward(d, E4, U3);
+    BW(9);
+    BW(8);
+    pool_backward(d, E3, P4);
+    BW(7);
+    BW(6);
+    pool_backward(d, E2, P3);
+    BW(5);
+    BW(4);
+    pool_backward(d, E1, P2);
+    BW(3);
+    BW(2);
+    pool_backward(d, E0, P1);
+    BW(1);
+#undef BW
+    return layer_backward(d, 0, false);
+}
+
+hipError_t pack_weights(ohpt_denoiser* d) {
+    for (Layer& L : d->layers) {
+        const float* w = d->params.as<float>() + L.w_off;
+        const int nf = L.nchunk * L.ks * L.np * 32;
+        k_pack_fwd<<<cdiv(nf, 256), 256, 0, d->stream>>>(w, L.wf.as<uint16_t>(), L.cin, L.cout, L.np, L.cc, L.ks, nf);
+        if (L.sp.in0 != X0) {
+            const int nd = (L.np / 32) * 9 * L.cin * 32;
+            k_pack_dgrad<<<cdiv(nd, 256), 256, 0, d->stream>>>(w, L.wd.as<uint16_t>(), L.cin, L.cout, nd);
+        }
+    }
+    return hipGetLastError();
+}
+
+// Stages the guide images (and target) on the device when given host pointers.
+struct Inputs { const float4* rad; const float4* alb; const float4* nrm; const float* dep; const float4* tgt; };
+
+int stage_inputs(ohpt_denoiser* d, const ohpt_denoise_inputs* in, const float* target, size_t npx, int on_device,
+                 Inputs& out) {
+    if (on_device) {
+        out = {reinterpret_cast<const float4*>(in->radiance), reinterpret_cast<const float4*>(in->albedo),
+               reinterpret_cast<const float4*>(in->normal), in->depth, reinterpret_cast<const float4*>(target)};
+        return OHPT_OK;
+    }
+    auto up = [&](DevBuf& buf, const void* src, size_t bytes) -> hipError_t {
+        hipError_t e = buf.ensure(bytes);
+        if (e == hipSuccess) e = hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, d->stream);
+        return e;
+    };
+    out = {};
+    DNCHK(up(d->in_rad, in->radiance, npx * 16));
+    out.rad = d->in_rad.as<float4>();
+    if (in->albedo) { DNCHK(up(d->in_alb, in->albedo, npx * 16)); out.alb = d->in_alb.as<float4>(); }
+    if (in->normal) { DNCHK(up(d->in_nrm, in->normal, npx * 16)); out.nrm = d->in_nrm.as<float4>(); }
+    if (in->depth) { DNCHK(up(d->in_dep, in->depth, npx * 4)); out.dep = d->in_dep.as<float>(); }
+    if (target) { DNCHK(up(d->in_tgt, target, npx * 16)); out.tgt = d->in_tgt.as<float4>(); }
+    return OHPT_OK;
+}
+
+int check_shape(ohpt_denoiser* d, const ohpt_denoise_inputs* in, uint32_t batch, uint32_t width, uint32_t height) {
+    if (!d || !in || !in->radiance) return set_error(OHPT_ERR_INVALID_ARG, "denoiser: null argument");
+    if (batch == 0 || width == 0 || height == 0 || batch > 1024 || width > 16384 || height > 16384)
+        return set_error(OHPT_ERR_INVALID_ARG, "denoiser: bad image size");
+    if ((uint64_t)batch * pad16((int)width) * pad16((int)height) > (1ull << 28))
+        return set_error(OHPT_ERR_INVALID_ARG, "denoiser: batch too large");
+    return OHPT_OK;
+}
+
+int run_forward(ohpt_denoiser* d, const Inputs& x, uint32_t batch, uint32_t width, uint32_t height) {
+    d->B = (int)batch;
+    d->H = (int)height;
+    d->W = (int)width;
+    d->Hp = pad16((int)height);
+    d->Wp = pad16((int)width);
+    DNCHK(ensure_capacity(d, d->B, d->Hp, d->Wp));
+    const size_t ppx = (size_t)d->B * d->Hp * d->Wp;
+    k_features<<<cdiv(ppx, 256), 256, 0, d->stream>>>(x.rad, x.alb, x.nrm, x.dep, d->act[X0].as<uint16_t>(), d->B,
+                                                      d->H, d->W, d->Hp, d->Wp);
+    network_forward(d);
+    DNCHK(hipGetLastError());
+    return OHPT_OK;
+}
+
+void begin_timing(ohpt_denoiser* d) {
+    d->conv_ev_used = 0;
+    (void)hipEventRecord(d->ev0, d->stream);
+}
+
+int end_timing(ohpt_denoiser* d, bool training) {
+    DNCHK(hipEventRecord(d->ev1, d->stream));
+    DNCHK(hipEventSynchronize(d->ev1));
+    float ms = 0.0f;
+    DNCHK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    d->stats.ms_total = ms;
+    float conv = 0.0f;
+    for (int i = 0; i + 1 < d->conv_ev_used; i += 2) {
+        float m = 0.0f;
+        if (hipEventElapsedTime(&m, d->conv_ev[i], d->conv_ev[i + 1]) == hipSuccess) conv += m;
+    }
+    d->stats.ms_conv = conv;
+    d->stats.conv_flops = conv_flops(d, d->H, d->W, training);
+    d->stats.padded_width = (uint32_t)d->Wp;
+    d->stats.padded_height = (uint32_t)d->Hp;
+    d->stats.batch = (uint32_t)d->B;
+    return OHPT_OK;
+}
+
+// Seeded normal variates (splitmix64 + Box-Muller) for the initial weights.
+struct InitRng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() {
+        const double u1 = uniform(), u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ohpt_denoiser_create(ohpt_denoiser** out, int device, uint64_t seed) {
+    if (!out) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_create: null output");
+    *out = nullptr;
+    int ndev = 0;
+    DNCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_create: no such device");
+    DNCHK(hipSetDevice(device));
+    auto* d = new ohpt_denoiser();
+    d->device = device;
+    size_t off = 0;
+    for (int i = 0; i < kNumLayers; ++i) {
+        Layer L{};
+        L.sp = kLayers[i];
+        L.cin0 = kActChan[L.sp.in0];
+        L.cin1 = L.sp.in1 >= 0 ? kActChan[L.sp.in1] : 0;
+        L.cin = L.cin0 + L.cin1;
+        L.cout = L.sp.cout;
+        L.np = (L.cout + 31) & ~31;
+        L.cc = (L.cin % 32 == 0) ? 32 : 16;
+        L.ks = (9 * L.cc + 31) / 32;
+        L.nchunk = L.cin / L.cc;
+        L.w_off = off;
+        off += (size_t)9 * L.cin * L.cout;
+        L.b_off = off;
+        off += L.cout;
+        d->layers.push_back(std::move(L));
+    }
+    d->nparam = off;
+    // initial parameters on the host
+    std::vector<float> p(d->nparam, 0.0f);
+    InitRng rng{seed ^ 0x6F68707464656E6Full};
+    for (const Layer& L : d->layers) {
+        const double fan_in = 9.0 * (L.sp.in0 == X0 ? kInputChannels : L.cin);
+        double std_ = std::sqrt(2.0 / fan_in);
+        if (L.sp.out == kLogits) std_ = 0.1 * std::sqrt(1.0 / fan_in);
+        for (int t = 0; t < 9; ++t)
+            for (int ci = 0; ci < L.cin; ++ci)
+                for (int co = 0; co < L.cout; ++co) {
+                    const bool pad_channel = (L.sp.in0 == X0 && ci >= kInputChannels);
+                    const double v = rng.normal() * std_;
+                    p[L.w_off + ((size_t)t * L.cin + ci) * L.cout + co] = pad_channel ? 0.0f : (float)v;
+                }
+    }
+    auto fail_out = [&](int rc) { ohpt_denoiser_destroy(d); return rc; };
+    hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&d->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&d->ev1);
+    if (e == hipSuccess) e = d->params.ensure(d->nparam * 4);
+    if (e == hipSuccess) e = d->grads.ensure(d->nparam * 4);
+    if (e == hipSuccess) e = d->adam_m.ensure(d->nparam * 4);
+    if (e == hipSuccess) e = d->adam_v.ensure(d->nparam * 4);
+    if (e == hipSuccess) e = d->loss_dev.ensure(16);
+    for (Layer& L : d->layers) {
+        if (e == hipSuccess) e = L.wf.ensure((size_t)L.nchunk * L.ks * L.np * 32 * 2);
+        if (e == hipSuccess && L.sp.in0 != X0) e = L.wd.ensure((size_t)(L.np / 32) * 9 * L.cin * 32 * 2);
+    }
+    if (e != hipSuccess) return fail_out(set_hip_error("ohpt_denoiser_create", e));
+    *out = d;
+    const int rc = ohpt_denoiser_set_params(d, p.data(), d->nparam);
+    if (rc != OHPT_OK) {
+        *out = nullptr;
+        return fail_out(rc);
+    }
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_destroy(ohpt_denoiser* d) {
+    if (!d) return OHPT_OK;
+    (void)hipSetDevice(d->device);
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    for (hipEvent_t e : d->conv_ev) (void)hipEventDestroy(e);
+    if (d->ev0) (void)hipEventDestroy(d->ev0);
+    if (d->ev1) (void)hipEventDestroy(d->ev1);
+    hipStream_t s = d->stream;
+    delete d;
+    if (s) (void)hipStreamDestroy(s);
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_param_count(ohpt_denoiser* d, uint64_t* n) {
+    if (!d || !n) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_param_count: null argument");
+    *n = d->nparam;
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_get_params(ohpt_denoiser* d, float* out, uint64_t n) {
+    if (!d || !out) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_get_params: null argument");
+    if (n != d->nparam) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_get_params: wrong count");
+    DNCHK(hipSetDevice(d->device));
+    DNCHK(hipMemcpyAsync(out, d->params.p, n * 4, hipMemcpyDeviceToHost, d->stream));
+    DNCHK(hipStreamSynchronize(d->stream));
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_set_params(ohpt_denoiser* d, const float* in, uint64_t n) {
+    if (!d || !in) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_set_params: null argument");
+    if (n != d->nparam) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_set_params: wrong count");
+    DNCHK(hipSetDevice(d->device));
+    DNCHK(hipMemcpyAsync(d->params.p, in, n * 4, hipMemcpyHostToDevice, d->stream));
+    DNCHK(hipMemsetAsync(d->adam_m.p, 0, n * 4, d->stream));
+    DNCHK(hipMemsetAsync(d->adam_v.p, 0, n * 4, d->stream));
+    DNCHK(hipMemsetAsync(d->grads.p, 0, n * 4, d->stream));
+    d->adam_t = 0;
+    d->have_grads = false;
+    DNCHK(pack_weights(d));
+    DNCHK(hipStreamSynchronize(d->stream));
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_get_grads(ohpt_denoiser* d, float* out, uint64_t n) {
+    if (!d || !out) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_get_grads: null argument");
+    if (n != d->nparam) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_get_grads: wrong count");
+    DNCHK(hipSetDevice(d->device));
+    DNCHK(hipMemcpyAsync(out, d->grads.p, n * 4, hipMemcpyDeviceToHost, d->stream));
+    DNCHK(hipStreamSynchronize(d->stream));
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_forward_backward(ohpt_denoiser* d, const ohpt_denoise_inputs* in, const float* target,
+                                   uint32_t batch, uint32_t width, uint32_t height, float* loss, int on_device) {
+    int rc = check_shape(d, in, batch, width, height);
+    if (rc != OHPT_OK) return rc;
+    if (!target) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_forward_backward: null target");
+    DNCHK(hipSetDevice(d->device));
+    Inputs x{};
+    rc = stage_inputs(d, in, target, (size_t)batch * width * height, on_device, x);
+    if (rc != OHPT_OK) return rc;
+    begin_timing(d);
+    rc = run_forward(d, x, batch, width, height);
+    if (rc != OHPT_OK) return rc;
+    const size_t ppx = (size_t)d->B * d->Hp * d->Wp;
+    KpnArgs k{};
+    k.logits = d->logits.as<float>();
+    k.rad = x.rad;
+    k.target = x.tgt;
+    k.dlogits = d->dlogits.as<uint16_t>();
+    k.loss_part = d->loss_part.as<float>();
+    k.B = d->B;
+    k.H = d->H;
+    k.W = d->W;
+    k.Hp = d->Hp;
+    k.Wp = d->Wp;
+    k.inv_count = (float)(1.0 / (3.0 * (double)batch * width * height));
+    const uint32_t nblk = cdiv(ppx, 256);
+    k_kpn<<<nblk, 256, 0, d->stream>>>(k);
+    k_sum_serial<<<1, 256, 0, d->stream>>>(d->loss_part.as<float>(), (int)nblk, k.inv_count, d->loss_dev.as<float>());
+    DNCHK(network_backward(d));
+    DNCHK(hipGetLastError());
+    rc = end_timing(d, true);
+    if (rc != OHPT_OK) return rc;
+    d->have_grads = true;
+    if (loss) {
+        DNCHK(hipMemcpyAsync(loss, d->loss_dev.p, 4, hipMemcpyDeviceToHost, d->stream));
+        DNCHK(hipStreamSynchronize(d->stream));
+    }
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_adam_step(ohpt_denoiser* d, float lr) {
+    if (!d) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_adam_step: null denoiser");
+    if (!d->have_grads) return set_error(OHPT_ERR_STATE, "ohpt_denoiser_adam_step: no gradient computed yet");
+    if (!(lr >= 0.0f)) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_adam_step: bad learning rate");
+    DNCHK(hipSetDevice(d->device));
+    d->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float bc1 = (float)(1.0 - std::pow(0.9, d->adam_t));
+    const float bc2 = (float)(1.0 - std::pow(0.999, d->adam_t));
+    k_adam<<<cdiv(d->nparam, 256), 256, 0, d->stream>>>(d->params.as<float>(), d->grads.as<float>(),
+                                                         d->adam_m.as<float>(), d->adam_v.as<float>(), d->nparam, lr,
+                                                         b1, b2, eps, bc1, bc2);
+    DNCHK(pack_weights(d));
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_train_step(ohpt_denoiser* d, const ohpt_denoise_inputs* in, const float* target, uint32_t batch,
+                             uint32_t width, uint32_t height, float lr, float* loss, int on_device) {
+    const int rc = ohpt_denoiser_forward_backward(d, in, target, batch, width, height, loss, on_device);
+    if (rc != OHPT_OK) return rc;
+    return ohpt_denoiser_adam_step(d, lr);
+}
+
+int ohpt_denoiser_infer(ohpt_denoiser* d, const ohpt_denoise_inputs* in, float* out, uint32_t batch, uint32_t width,
+                        uint32_t height, int on_device) {
+    int rc = check_shape(d, in, batch, width, height);
+    if (rc != OHPT_OK) return rc;
+    if (!out) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_infer: null output");
+    DNCHK(hipSetDevice(d->device));
+    const size_t npx = (size_t)batch * width * height;
+    Inputs x{};
+    rc = stage_inputs(d, in, nullptr, npx, on_device, x);
+    if (rc != OHPT_OK) return rc;
+    float4* dst = reinterpret_cast<float4*>(out);
+    if (!on_device) {
+        DNCHK(d->out_stage.ensure(npx * 16));
+        dst = d->out_stage.as<float4>();
+    }
+    begin_timing(d);
+    rc = run_forward(d, x, batch, width, height);
+    if (rc != OHPT_OK) return rc;
+    KpnArgs k{};
+    k.logits = d->logits.as<float>();
+    k.rad = x.rad;
+    k.out = dst;
+    k.B = d->B;
+    k.H = d->H;
+    k.W = d->W;
+    k.Hp = d->Hp;
+    k.Wp = d->Wp;
+    k_kpn<<<cdiv((size_t)d->B * d->Hp * d->Wp, 256), 256, 0, d->stream>>>(k);
+    DNCHK(hipGetLastError());
+    rc = end_timing(d, false);
+    if (rc != OHPT_OK) return rc;
+    if (!on_device) {
+        DNCHK(hipMemcpyAsync(out, dst, npx * 16, hipMemcpyDeviceToHost, d->stream));
+        DNCHK(hipStreamSynchronize(d->stream));
+    }
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_read_tensor(ohpt_denoiser* d, uint32_t index, float* out, uint64_t n) {
+    if (!d || !out) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_read_tensor: null argument");
+    if (d->B == 0) return set_error(OHPT_ERR_STATE, "ohpt_denoiser_read_tensor: nothing computed yet");
+    DNCHK(hipSetDevice(d->device));
+    const size_t px = (size_t)d->B * d->Hp * d->Wp;
+    const bool grad = index >= 100;
+    const uint32_t t = grad ? index - 100 : index;
+    if (t == (uint32_t)kLogits) {
+        const size_t cnt = px * 32;
+        if (n != cnt) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_read_tensor: wrong count");
+        if (!grad) {
+            DNCHK(hipMemcpyAsync(out, d->logits.p, cnt * 4, hipMemcpyDeviceToHost, d->stream));
+            DNCHK(hipStreamSynchronize(d->stream));
+            return OHPT_OK;
+        }
+        DevBuf tmp;
+        DNCHK(tmp.ensure(cnt * 4));
+        k_bf16_to_f32<<<cdiv(cnt, 256), 256, 0, d->stream>>>(d->dlogits.as<uint16_t>(), tmp.as<float>(), cnt);
+        DNCHK(hipMemcpyAsync(out, tmp.p, cnt * 4, hipMemcpyDeviceToHost, d->stream));
+        DNCHK(hipStreamSynchronize(d->stream));
+        return OHPT_OK;
+    }
+    if (t >= (uint32_t)kNumAct || (grad && t == (uint32_t)X0))
+        return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_read_tensor: no such tensor");
+    const size_t cnt = (px >> (2 * kActLevel[t])) * kActChan[t];
+    if (n != cnt) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_read_tensor: wrong count");
+    DevBuf tmp;
+    DNCHK(tmp.ensure(cnt * 4));
+    const DevBuf& src = grad ? d->gact[t] : d->act[t];
+    k_bf16_to_f32<<<cdiv(cnt, 256), 256, 0, d->stream>>>(src.as<uint16_t>(), tmp.as<float>(), cnt);
+    DNCHK(hipMemcpyAsync(out, tmp.p, cnt * 4, hipMemcpyDeviceToHost, d->stream));
+    DNCHK(hipStreamSynchronize(d->stream));
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_set_profiling(ohpt_denoiser* d, int on) {
+    if (!d) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_set_profiling: null denoiser");
+    d->profile = on != 0;
+    return OHPT_OK;
+}
+
+int ohpt_denoiser_get_stats(ohpt_denoiser* d, ohpt_denoiser_stats* out) {
+    if (!d || !out) return set_error(OHPT_ERR_INVALID_ARG, "ohpt_denoiser_get_stats: null argument");
+    *out = d->stats;
+    return OHPT_OK;
+}
+
+}  // extern "C"
+
+namespace ohpt {
+// Device-side entry used by ohpt_denoise: inputs already resolved on the device.
+int denoiser_infer_device(ohpt_denoiser* d, const float* rad, const float* alb, const float* nrm, const float* dep,
+                          float* out, uint32_t width, uint32_t height) {
+    ohpt_denoise_inputs in{rad, alb, nrm, dep};
+    return ohpt_denoiser_infer(d, &in, out, 1, width, height, 1);
+}
+}  // namespace ohpt

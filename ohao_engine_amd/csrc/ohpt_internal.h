@@ -117,4 +117,46 @@ void launch_resolve_aovs(const float4* accum, const float4* aov_albedo, const fl
 hipError_t env_cdf_build(const float4* env, uint32_t W, uint32_t H, float* marg, float* cond,
                          float* row_sum, float* integral_dev, hipStream_t s);
 
+// ---- denoiser (denoiser.hip) ---------------------------------------------------
+// Inference on device-resident guides (RGBA32F radiance/albedo/normal, F32
+// depth; albedo/normal/depth may be null) into device RGBA32F `out`; returns
+// an OHPT_* status. Synchronous.
+int denoiser_infer_device(ohpt_denoiser* d, const float* rad, const float* alb, const float* nrm, const float* dep,
+                          float* out, uint32_t width, uint32_t height);
+
+// ---- host helpers shared by the C-ABI translation units --------------------
+// Thread-local last-error message behind ohpt_last_error(); returns `code`.
+int set_error(int code, const char* msg);
+int set_hip_error(const char* what, hipError_t e);
+
+// Owning device allocation that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
 }  // namespace ohpt

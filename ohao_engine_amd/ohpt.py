@@ -44,8 +44,21 @@ class Stats(C.Structure):
                 ("tri_tests", C.c_uint64), ("extend_launches", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class DenoiserStats(C.Structure):
+    _fields_ = [("ms_total", C.c_float), ("ms_conv", C.c_float), ("conv_flops", C.c_double),
+                ("padded_width", C.c_uint32), ("padded_height", C.c_uint32), ("batch", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
 # (name, restype, argtypes) of every entry point declared in include/ohpt.h
 _vp, _f32p, _u32p, _u8p = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)
+
+
+class DenoiseInputs(C.Structure):
+    _fields_ = [("radiance", _f32p), ("albedo", _f32p), ("normal", _f32p), ("depth", _f32p)]
+
+
+_dinp = C.POINTER(DenoiseInputs)
 SIGNATURES = [
     ("ohpt_create", C.c_int, [C.POINTER(_vp), C.c_int, C.c_uint32, C.c_uint32]),
     ("ohpt_destroy", C.c_int, [_vp]),
@@ -84,6 +97,22 @@ SIGNATURES = [
     ("ohpt_read_texture_gradient", C.c_int, [_vp, C.c_int32, _f32p, C.c_int]),
     ("ohpt_trace_rays", C.c_int, [_vp, _f32p, _f32p, C.c_uint32, C.c_int]),
     ("ohpt_occluded_rays", C.c_int, [_vp, _f32p, _u32p, C.c_uint32, C.c_int]),
+    ("ohpt_denoiser_create", C.c_int, [C.POINTER(_vp), C.c_int, C.c_uint64]),
+    ("ohpt_denoiser_destroy", C.c_int, [_vp]),
+    ("ohpt_denoiser_param_count", C.c_int, [_vp, C.POINTER(C.c_uint64)]),
+    ("ohpt_denoiser_get_params", C.c_int, [_vp, _f32p, C.c_uint64]),
+    ("ohpt_denoiser_set_params", C.c_int, [_vp, _f32p, C.c_uint64]),
+    ("ohpt_denoiser_get_grads", C.c_int, [_vp, _f32p, C.c_uint64]),
+    ("ohpt_denoiser_forward_backward", C.c_int, [_vp, _dinp, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, _f32p,
+                                                 C.c_int]),
+    ("ohpt_denoiser_adam_step", C.c_int, [_vp, C.c_float]),
+    ("ohpt_denoiser_train_step", C.c_int, [_vp, _dinp, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                           _f32p, C.c_int]),
+    ("ohpt_denoiser_infer", C.c_int, [_vp, _dinp, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]),
+    ("ohpt_denoiser_read_tensor", C.c_int, [_vp, C.c_uint32, _f32p, C.c_uint64]),
+    ("ohpt_denoiser_get_stats", C.c_int, [_vp, C.POINTER(DenoiserStats)]),
+    ("ohpt_denoiser_set_profiling", C.c_int, [_vp, C.c_int]),
+    ("ohpt_denoise", C.c_int, [_vp, _vp, _f32p, C.c_int]),
 ]
 
 _lib = None
@@ -310,6 +339,21 @@ class Context:
         _check(self.L.ohpt_read_aovs(self.h, _fp(a), _fp(n), _fp(d), 0), "ohpt_read_aovs")
         return a, n, d
 
+    def read_aovs_device(self, albedo, normal, depth):
+        """First-hit AOVs into device tensors (any may be None)."""
+        ptr = lambda t: None if t is None else C.cast(t.data_ptr(), _f32p)
+        _check(self.L.ohpt_read_aovs(self.h, ptr(albedo), ptr(normal), ptr(depth), 1), "ohpt_read_aovs")
+
+    def denoise(self, dn: "Denoiser", out=None):
+        """Learned denoise of the current accumulation -> (H, W, 4) RGBA32F
+        (numpy), or into the device tensor `out`."""
+        if out is not None and hasattr(out, "data_ptr"):
+            _check(self.L.ohpt_denoise(self.h, dn.h, C.cast(out.data_ptr(), _f32p), 1), "ohpt_denoise")
+            return out
+        res = np.zeros((self.height, self.width, 4), np.float32)
+        _check(self.L.ohpt_denoise(self.h, dn.h, _fp(res), 0), "ohpt_denoise")
+        return res
+
     # -- differentiation ---------------------------------------------------------
     def zero_gradients(self):
         _check(self.L.ohpt_zero_gradients(self.h), "ohpt_zero_gradients")
@@ -348,3 +392,171 @@ def context_for(sc: "S.Scene", device: int = 0) -> Context:
     ctx = Context(sc.width, sc.height, device)
     ctx.load_scene(sc)
     return ctx
+
+
+# ---------------------------------------------------------------- denoiser
+# activation tensors of the network in evaluation order: (name, level, channels)
+DENOISER_TENSORS = [
+    ("x0", 0, 16), ("a0", 0, 32), ("e0", 0, 32), ("p1", 1, 32), ("a1", 1, 64), ("e1", 1, 64),
+    ("p2", 2, 64), ("a2", 2, 96), ("e2", 2, 96), ("p3", 3, 96), ("a3", 3, 128), ("e3", 3, 128),
+    ("p4", 4, 128), ("a4", 4, 160), ("e4", 4, 160), ("u3", 3, 160), ("b3", 3, 128), ("d3", 3, 128),
+    ("u2", 2, 128), ("b2", 2, 96), ("d2", 2, 96), ("u1", 1, 96), ("b1", 1, 64), ("d1", 1, 64),
+    ("u0", 0, 64), ("b0", 0, 32), ("d0", 0, 32)]
+TENSOR_LOGITS = 27
+TENSOR_GRAD = 100
+
+
+def _pad16(v: int) -> int:
+    return (v + 15) & ~15
+
+
+class Denoiser:
+    """Kernel-predicting U-Net denoiser (ohpt_denoiser_*). Images are
+    (B, H, W, 4) RGBA32F radiance / albedo / normal and (B, H, W) depth, as
+    numpy arrays (host) or torch tensors on the GPU (zero copy)."""
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        _check(self.L.ohpt_denoiser_create(C.byref(h), device, seed), "ohpt_denoiser_create")
+        self.h = h
+        n = C.c_uint64()
+        _check(self.L.ohpt_denoiser_param_count(self.h, C.byref(n)), "ohpt_denoiser_param_count")
+        self.nparam = int(n.value)
+        self._shape = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ohpt_denoiser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- parameters -------------------------------------------------------------
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.nparam, np.float32)
+        _check(self.L.ohpt_denoiser_get_params(self.h, _fp(out), self.nparam), "ohpt_denoiser_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32).reshape(-1)
+        _check(self.L.ohpt_denoiser_set_params(self.h, _fp(p), p.size), "ohpt_denoiser_set_params")
+
+    def get_grads(self) -> np.ndarray:
+        out = np.zeros(self.nparam, np.float32)
+        _check(self.L.ohpt_denoiser_get_grads(self.h, _fp(out), self.nparam), "ohpt_denoiser_get_grads")
+        return out
+
+    def save(self, path):
+        p = self.get_params()
+        header = np.array([0x4F48444E, 1, self.nparam], np.uint32)   # "OHDN", version, count
+        with open(path, "wb") as f:
+            f.write(header.tobytes())
+            f.write(p.tobytes())
+
+    def load(self, path):
+        raw = np.fromfile(path, np.uint8)
+        header = raw[:12].view(np.uint32)
+        if header[0] != 0x4F48444E or header[1] != 1 or header[2] != self.nparam:
+            raise OhptError(f"{path}: not a denoiser parameter file for this network")
+        self.set_params(raw[12:].view(np.float32))
+
+    # -- evaluation -------------------------------------------------------------
+    @staticmethod
+    def _inputs(radiance, albedo, normal, depth):
+        keep = []
+        dev = hasattr(radiance, "data_ptr")
+
+        def ptr(a, shape_last):
+            if a is None:
+                return None
+            if dev:
+                return C.cast(a.data_ptr(), _f32p)
+            arr = np.ascontiguousarray(a, np.float32)
+            keep.append(arr)
+            return arr.ctypes.data_as(_f32p)
+
+        inp = DenoiseInputs(ptr(radiance, 4), ptr(albedo, 4), ptr(normal, 4), ptr(depth, 1))
+        return inp, keep, dev
+
+    @staticmethod
+    def _bhw(radiance):
+        shp = tuple(radiance.shape)
+        if len(shp) == 3:
+            return 1, shp[0], shp[1]
+        return shp[0], shp[1], shp[2]
+
+    def infer(self, radiance, albedo=None, normal=None, depth=None, out=None):
+        B, H, W = self._bhw(radiance)
+        inp, keep, dev = self._inputs(radiance, albedo, normal, depth)
+        if dev:
+            if out is None:
+                import torch
+                out = torch.empty((B, H, W, 4), dtype=torch.float32, device=radiance.device)
+            _check(self.L.ohpt_denoiser_infer(self.h, C.byref(inp), C.cast(out.data_ptr(), _f32p), B, W, H, 1),
+                   "ohpt_denoiser_infer")
+        else:
+            out = np.zeros((B, H, W, 4), np.float32)
+            _check(self.L.ohpt_denoiser_infer(self.h, C.byref(inp), _fp(out), B, W, H, 0), "ohpt_denoiser_infer")
+        self._shape = (B, H, W)
+        return out
+
+    def forward_backward(self, radiance, albedo, normal, depth, target) -> float:
+        B, H, W = self._bhw(radiance)
+        inp, keep, dev = self._inputs(radiance, albedo, normal, depth)
+        loss = C.c_float()
+        if dev:
+            tp = C.cast(target.data_ptr(), _f32p)
+        else:
+            t = np.ascontiguousarray(target, np.float32)
+            keep.append(t)
+            tp = _fp(t)
+        _check(self.L.ohpt_denoiser_forward_backward(self.h, C.byref(inp), tp, B, W, H, C.byref(loss),
+                                                      1 if dev else 0), "ohpt_denoiser_forward_backward")
+        self._shape = (B, H, W)
+        return float(loss.value)
+
+    def adam_step(self, lr: float):
+        _check(self.L.ohpt_denoiser_adam_step(self.h, lr), "ohpt_denoiser_adam_step")
+
+    def train_step(self, radiance, albedo, normal, depth, target, lr: float) -> float:
+        B, H, W = self._bhw(radiance)
+        inp, keep, dev = self._inputs(radiance, albedo, normal, depth)
+        loss = C.c_float()
+        if dev:
+            tp = C.cast(target.data_ptr(), _f32p)
+        else:
+            t = np.ascontiguousarray(target, np.float32)
+            keep.append(t)
+            tp = _fp(t)
+        _check(self.L.ohpt_denoiser_train_step(self.h, C.byref(inp), tp, B, W, H, lr, C.byref(loss),
+                                                1 if dev else 0), "ohpt_denoiser_train_step")
+        self._shape = (B, H, W)
+        return float(loss.value)
+
+    def read_tensor(self, index: int) -> np.ndarray:
+        """Activation `index` (or TENSOR_GRAD + index for its gradient) of the
+        last call as (B, Hp >> level, Wp >> level, C) float32."""
+        B, H, W = self._shape
+        Hp, Wp = _pad16(H), _pad16(W)
+        t = index - TENSOR_GRAD if index >= TENSOR_GRAD else index
+        if t == TENSOR_LOGITS:
+            shape = (B, Hp, Wp, 32)
+        else:
+            _, lvl, ch = DENOISER_TENSORS[t]
+            shape = (B, Hp >> lvl, Wp >> lvl, ch)
+        out = np.zeros(shape, np.float32)
+        _check(self.L.ohpt_denoiser_read_tensor(self.h, index, _fp(out), out.size), "ohpt_denoiser_read_tensor")
+        return out
+
+    def set_profiling(self, on: bool):
+        _check(self.L.ohpt_denoiser_set_profiling(self.h, 1 if on else 0), "ohpt_denoiser_set_profiling")
+
+    def stats(self) -> DenoiserStats:
+        st = DenoiserStats()
+        _check(self.L.ohpt_denoiser_get_stats(self.h, C.byref(st)), "ohpt_denoiser_get_stats")
+        return st
