@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -40,6 +41,138 @@ def extend_bytes(rays, visits, tests):
     """Algorithmic bytes of the extend kernel: 32 B ray + 16 B hit per ray,
     64 B per node visit, 48 B per triangle test."""
     return 48.0 * rays + 64.0 * visits + 48.0 * tests
+
+
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak
+
+
+def equal_spp(plain, target):
+    """Plain spp whose relMSE equals `target`: log-log interpolation between the
+    measured checkpoints, slope -1 (MSE ~ 1/spp) outside them."""
+    xs = sorted(plain)
+    ls = [math.log(plain[c]["relmse"]) for c in xs]
+    lt = math.log(target)
+    if lt >= ls[0]:
+        return xs[0] * math.exp(ls[0] - lt)
+    for i in range(len(xs) - 1):
+        if ls[i] >= lt >= ls[i + 1]:
+            f = (ls[i] - lt) / max(ls[i] - ls[i + 1], 1e-12)
+            return math.exp(math.log(xs[i]) + f * (math.log(xs[i + 1]) - math.log(xs[i])))
+    return xs[-1] * math.exp(ls[-1] - lt)
+
+
+def ldr(img):
+    """Display transform of the RGBA8 readback, (ACES(0.5 x))^(1/2.2) in [0, 1]."""
+    u = 0.5 * np.maximum(np.asarray(img, np.float64)[..., :3], 0.0)
+    a = np.clip((u * (2.51 * u + 0.03)) / (u * (2.43 * u + 0.59) + 0.14), 0.0, 1.0)
+    return a ** (1.0 / 2.2)
+
+
+def ldr_rmse(x, ref):
+    return float(np.sqrt(np.mean((ldr(x) - ldr(ref)) ** 2)))
+
+
+def save_png(path, img):
+    import struct
+    import zlib
+    v = (np.clip(ldr(img), 0, 1) * 255 + 0.5).astype(np.uint8)
+    h, w = v.shape[:2]
+    raw = b"".join(b"\x00" + v[y].tobytes() for y in range(h))
+    ch = lambda t, d: struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    Path(path).write_bytes(b"\x89PNG\r\n\x1a\n" + ch(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0))
+                           + ch(b"IDAT", zlib.compress(raw, 6)) + ch(b"IEND", b""))
+
+
+def denoise_leg(ctx, sc, ref_spp, save_dir=None):
+    """GPU time to equal relMSE: the learned denoiser on a 1-spp render against
+    plain progressive accumulation, both scored against a held-out high-spp
+    render of the benchmark view (a scene seed never used for training). The
+    reference is two independent halves, whose disagreement estimates its own
+    noise floor."""
+    from ohao_engine_amd import ohpt, denoise as D
+    if not D.WEIGHTS.exists():
+        log("denoise: no trained weights in the package, leg skipped")
+        return None
+    if save_dir:
+        Path(save_dir).mkdir(parents=True, exist_ok=True)
+    t0 = time.time()
+    halves = []
+    for seed in (50_000_000, 60_000_000):
+        ctx.set_render_seed(seed)
+        ctx.reset()
+        for k in range(0, ref_spp // 2, 512):
+            ctx.render(min(512, ref_spp // 2 - k))
+        halves.append(ctx.read_accum())
+        log(f"denoise: reference half {len(halves)}/2 ({ref_spp // 2} spp, {time.time() - t0:.0f}s)")
+    ref = 0.5 * (halves[0] + halves[1])
+    ref[..., 3] = 1.0
+    # relMSE(half A, half B) / 4 estimates relMSE(truth, ref) for the averaged reference
+    floor = 0.25 * D.relmse(halves[0], halves[1])
+    ld = D.LearnedDenoiser(D.WEIGHTS)
+    untrained = ohpt.Denoiser(seed=1)
+    ctx.set_render_seed(1)
+    ctx.reset()
+    plain, done, gpu_ms = {}, 0, 0.0
+    den = None
+    for c in (1, 4, 16, 64, 256, 1024):
+        ctx.render(c - done, ohpt.RENDER_PROFILE)
+        gpu_ms += ctx.stats().ms_total
+        done = c
+        img = ctx.read_accum()
+        plain[c] = {"relmse": D.relmse(img, ref), "ldr_rmse": ldr_rmse(img, ref), "gpu_ms": round(gpu_ms, 3)}
+        if save_dir and c in (1, 1024):
+            save_png(Path(save_dir) / f"plain_{c}spp.png", img)
+        if c == 1:
+            ctx.denoise(ld.dn)                      # warm-up
+            times = []
+            for _ in range(5):
+                den = ctx.denoise(ld.dn)
+                times.append(ld.dn.stats().ms_total)
+            ld.dn.set_profiling(True)
+            ctx.denoise(ld.dn)
+            st = ld.dn.stats()
+            ld.dn.set_profiling(False)
+            ms_conv, conv_flops = st.ms_conv, st.conv_flops
+            ms_den = float(np.median(times))
+            alb, nrm, dep = ctx.read_aovs()
+            gate = ld.validate((img[None], alb[None], nrm[None], dep[None]), ref)
+            und = untrained.infer(img, alb, nrm, dep)[0]
+            e_untrained, l_untrained = D.relmse(und, ref), ldr_rmse(und, ref)
+            if save_dir:
+                save_png(Path(save_dir) / "denoised_1spp.png", den)
+                save_png(Path(save_dir) / "reference.png", ref)
+        log(f"denoise: plain {c} spp relMSE {plain[c]['relmse']:.5f} LDR RMSE {plain[c]['ldr_rmse']:.5f} "
+            f"({gpu_ms:.1f} ms GPU)")
+    untrained.close()
+    e_den, l_den = D.relmse(den, ref), ldr_rmse(den, ref)
+    spp_eq = equal_spp(plain, e_den)
+    ms_per_spp = plain[1024]["gpu_ms"] / 1024.0
+    t_den = plain[1]["gpu_ms"] + ms_den
+    t_plain = spp_eq * ms_per_spp
+    ld.close()
+    achieved = conv_flops / (ms_conv * 1e-3) / 1e12 if ms_conv > 0 else 0.0
+    res = {"metric": "GPU time to equal relMSE, denoised 1 spp vs plain PT",
+           "reference": f"{ref_spp} spp as two independent halves (render seeds 50000000, 60000000)",
+           "reference_noise_floor_relmse": round(floor, 6), "relmse_eps": D.RELMSE_EPS,
+           "relmse_plain": {str(c): round(v["relmse"], 6) for c, v in plain.items()},
+           "ldr_rmse_plain": {str(c): round(v["ldr_rmse"], 6) for c, v in plain.items()},
+           "gpu_ms_plain": {str(c): v["gpu_ms"] for c, v in plain.items()},
+           "relmse_denoised": round(e_den, 6), "ldr_rmse_denoised": round(l_den, 6),
+           "relmse_untrained_network": round(e_untrained, 6), "ldr_rmse_untrained_network": round(l_untrained, 6),
+           "gpu_ms_denoised": round(t_den, 3), "denoise_ms": round(ms_den, 3),
+           "reaches_plain_1024_relmse": bool(e_den <= plain[1024]["relmse"]),
+           "gpu_time_ratio_vs_plain_1024": round(plain[1024]["gpu_ms"] / t_den, 2),
+           "plain_spp_equal": round(spp_eq, 2), "plain_spp_equal_extrapolated": bool(spp_eq > 1024),
+           "gpu_ms_plain_equal": round(t_plain, 3), "speedup_at_equal_relmse": round(t_plain / t_den, 2),
+           "gate": {"enabled": gate.enabled, "bound": gate.bound},
+           "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 5),
+                        "kernel": "k_conv3x3 (BF16 MFMA implicit GEMM, 19 convolutions)",
+                        "conv_gflop_per_inference": round(conv_flops / 1e9, 2), "conv_ms": round(ms_conv, 3)}}
+    log(f"denoise: relMSE denoised {e_den:.5f} (untrained {e_untrained:.5f}, 1 spp {plain[1]['relmse']:.5f}, "
+        f"1024 spp {plain[1024]['relmse']:.5f}, reference floor ~{floor:.5f}); LDR RMSE denoised {l_den:.5f} "
+        f"vs 1024 spp {plain[1024]['ldr_rmse']:.5f}; GPU {t_den:.2f} ms vs {plain[1024]['gpu_ms']:.1f} ms")
+    return res
 
 
 def cpu_baseline(sc, seconds, threads):
@@ -70,6 +203,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-denoise", action="store_true")
+    ap.add_argument("--reference-spp", type=int, default=4096)
+    ap.add_argument("--save-images", default=None, help="directory for PNGs of the denoise comparison")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -152,6 +288,10 @@ def main():
                "cost_vs_forward": round((adj_wall / adj_steps) / (wall / a.steps), 3),
                "parameters": "1024x512 RGB ground albedo texture", "mode": "fp32 atomics"}
     log(f"adjoint: {adjoint['ms_per_step']} ms/step, {adjoint['cost_vs_forward']}x forward")
+
+    denoise = None
+    if not a.no_denoise:
+        denoise = denoise_leg(ctx, sc, a.reference_spp, a.save_images)
     ctx.close()
 
     cpu = None
@@ -171,7 +311,12 @@ def main():
                                   "NEE+MIS, PCG32 seed 7", "triangles": sc.num_triangles, "width": sc.width,
                       "height": sc.height, "spp_per_step": spp, "max_bounces": sc.settings.max_bounces,
                       "lbvh_build_ms": round(build_ms, 3)},
-           "breakdown": breakdown, "adjoint": adjoint, "roofline": roofline, "cpu_baseline": cpu}
+           "breakdown": breakdown, "adjoint": adjoint, "denoise": denoise, "roofline": roofline,
+           "cpu_baseline": cpu}
+    if denoise and cpu:
+        # CPU time for plain accumulation to reach the denoised image's relMSE
+        paths = sc.width * sc.height * denoise["plain_spp_equal"]
+        denoise["cpu_plain_seconds_to_equal_relmse"] = round(paths / (cpu["value"] * 1e6), 2)
     print(json.dumps(out), flush=True)
 
 

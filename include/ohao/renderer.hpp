@@ -1,0 +1,193 @@
+// C++ façade over the ohpt C-ABI with the call surface the reference's example
+// drivers use on VulkanRenderer (ohao/gpu/vulkan/renderer.hpp:114-292) and
+// Camera (ohao/render/camera/camera.hpp). Only the offline path tracer is
+// behind it: RenderMode::RTOffline (and RTRealtime, served by the same
+// integrator); Forward/Deferred are not provided.
+#pragma once
+
+#include <cstdint>
+#include <span>
+#include <string>
+#include <string_view>
+#include <vector>
+
+#include "../ohpt.h"
+#include "math.hpp"
+#include "scene.hpp"
+
+namespace ohao {
+
+// renderer.hpp:34-39
+enum class RenderMode { Forward, Deferred, RTRealtime, RTOffline };
+
+// denoise_types.hpp:13-19 plus Learned (the kernel-predicting network of
+// ohpt_denoiser_*). OIDN requests are served by Learned.
+enum class DenoiseMode : uint32_t { None = 0, OIDN = 1, NRD = 3, Atrous = 4, DLSSRR = 5, Learned = 6 };
+DenoiseMode parseDenoiseMode(std::string_view s);
+const char* denoiseModeName(DenoiseMode m);
+
+// camera.hpp / camera.cpp:16-44: position, pitch/yaw in degrees, vertical fov.
+class Camera {
+public:
+    Camera() { update(); }
+    void setPosition(vec3 p) { position_ = p; }
+    vec3 getPosition() const { return position_; }
+    void setRotation(float pitch, float yaw) {
+        pitch_ = pitch;
+        yaw_ = yaw;
+        update();
+    }
+    float getPitch() const { return pitch_; }
+    float getYaw() const { return yaw_; }
+    void setFov(float deg) { fov_ = deg; }
+    float getFov() const { return fov_; }
+    vec3 getFront() const { return front_; }
+    vec3 getRight() const { return right_; }
+    vec3 getUp() const { return up_; }
+    ohpt_camera toOhpt() const;
+
+private:
+    void update();
+    vec3 position_{0.0f, 0.0f, 2.5f};
+    float pitch_ = 0.0f, yaw_ = -90.0f, fov_ = 45.0f;
+    vec3 front_, right_, up_;
+};
+
+// Offline render settings (the RTRenderSettings fields of the offline path).
+struct RTRenderSettings {
+    uint32_t maxBounces = 4;
+    uint32_t rrStart = 2;
+    bool sobol = true;             // padded Owen-scrambled Sobol; false = PCG32
+    bool fireflyClamp = false;
+    float fireflyClampValue = 10.0f;
+};
+
+class Renderer {
+public:
+    Renderer(uint32_t width, uint32_t height, int device = 0);
+    ~Renderer();
+    Renderer(const Renderer&) = delete;
+    Renderer& operator=(const Renderer&) = delete;
+
+    // Lifecycle: initialize() creates the HIP context; false + message on failure.
+    [[nodiscard]] bool initialize();
+    void shutdown();
+    const std::string& lastError() const { return err_; }
+
+    // One render() traces one sample per pixel. The reference's frame ring
+    // returns pixels three frames late (render_dispatch.cpp:139-141), so
+    // `spp + 3` render() calls after resetAccumulation() hold exactly `spp`
+    // samples; the façade keeps that count: the first `frameLatency()`
+    // render() calls after a reset trace nothing. setFrameLatency(0) makes
+    // every call trace.
+    void render();
+    void setFrameLatency(uint32_t frames) { latency_ = frames; }
+    uint32_t frameLatency() const { return latency_; }
+    // Traces `spp` samples now (no latency emulation).
+    bool renderSamples(uint32_t spp);
+    void resize(uint32_t width, uint32_t height);
+
+    void setScene(Scene* scene);
+    Scene* getScene() const { return scene_; }
+    [[nodiscard]] bool updateSceneBuffers();
+    // Material / light edits without a BVH rebuild.
+    [[nodiscard]] bool updateRTMaterialParams();
+    [[nodiscard]] bool updateRTLightParams();
+    // Texel edits of scene textures (e.g. after an inverse-rendering step).
+    [[nodiscard]] bool updateTexture(int32_t id);
+
+    Camera& getCamera() { return camera_; }
+    const Camera& getCamera() const { return camera_; }
+
+    void setRenderMode(RenderMode mode) { mode_ = mode; }
+    RenderMode getRenderMode() const { return mode_; }
+    void setRTRenderSettings(const RTRenderSettings& s);
+    const RTRenderSettings& getRTRenderSettings() const { return settings_; }
+
+    // Radiance .hdr (RGBE) or PFM equirect; empty path clears the environment.
+    void setEnvironmentMap(std::string_view path);
+    // RGBA32F equirect supplied directly (row 0 = +Y).
+    void setEnvironmentPixels(std::vector<float> rgba, uint32_t width, uint32_t height);
+    void setEnvIntensityScale(float s);
+    float getEnvIntensityScale() const { return env_scale_; }
+
+    void resetAccumulation();
+    void setRenderSeed(uint32_t seed);
+    void notifyCameraChanged() { resetAccumulation(); }
+
+    // RGBA8 (ACES(x*0.5))^(1/2.2); denoised when the denoise mode is on and
+    // the denoiser's quality gate is open.
+    [[nodiscard]] const uint8_t* getPixels() const;
+    [[nodiscard]] std::span<const uint8_t> getPixelSpan() const {
+        const uint8_t* p = getPixels();
+        if (!p) return {};
+        return {p, pixels_.size()};
+    }
+    // Mean HDR beauty, first-hit albedo and normal, RGBA32F.
+    [[nodiscard]] bool readbackHDRBuffers(std::vector<float>& beauty, std::vector<float>& albedo,
+                                          std::vector<float>& normal, uint32_t& w, uint32_t& h);
+    uint32_t getWidth() const { return W_; }
+    uint32_t getHeight() const { return H_; }
+    uint32_t getSampleCount() const { return samples_; }
+
+    void setDenoiseMode(DenoiseMode mode);
+    DenoiseMode getDenoiseMode() const { return denoise_; }
+    // Learned denoiser parameters (ohpt_denoiser_*); default: the package's
+    // trained weights next to libohpt.so (weights/kpn_unet.bin).
+    bool loadDenoiserWeights(const std::string& path);
+    // Quality gate: render a `reference_spp` reference of the current view,
+    // compare the denoised `input_spp` image by relMSE; the denoiser stays on
+    // only if relMSE <= bound and below the plain image's. Returns the gate.
+    bool validateDenoiser(uint32_t input_spp, uint32_t reference_spp, float bound, float* relmse_out = nullptr);
+    bool denoiserGateOpen() const { return gate_open_; }
+
+    // Path-replay adjoint of the current image (dL/d mean radiance, RGBA32F)
+    // over `spp` samples from the render seed; texel gradients of scene
+    // textures accumulate until zeroGradients().
+    bool adjoint(const std::vector<float>& dl_dimage, uint32_t spp, bool deterministic = false);
+    bool zeroGradients();
+    bool textureGradient(int32_t id, std::vector<float>& out);
+
+    ohpt_context* handle() const { return ctx_; }
+    ohpt_stats lastStats() const;
+
+private:
+    bool fail(const std::string& what) const;
+    bool upload_lights();
+    bool upload_materials();
+    bool ensure_scene() const;
+    bool ensure_denoiser();
+
+    uint32_t W_, H_;
+    int device_;
+    ohpt_context* ctx_ = nullptr;
+    Scene* scene_ = nullptr;
+    bool scene_dirty_ = true;
+    Camera camera_;
+    RenderMode mode_ = RenderMode::RTOffline;
+    RTRenderSettings settings_;
+    float env_scale_ = 1.0f;
+    std::string env_path_, env_loaded_;
+    std::vector<float> env_pixels_;
+    uint32_t env_w_ = 0, env_h_ = 0;
+    bool env_dirty_ = false;
+    uint32_t seed_ = 0, samples_ = 0, latency_ = 3, since_reset_ = 0;
+    std::vector<int32_t> tex_ids_;       // scene texture id -> ohpt texture id
+    std::vector<uint32_t> actor_mat_;    // material slot per mesh actor, in upload order
+    DenoiseMode denoise_ = DenoiseMode::None;
+    ohpt_denoiser* dn_ = nullptr;
+    std::string dn_weights_;
+    bool gate_open_ = true;
+    mutable std::vector<uint8_t> pixels_;
+    mutable bool pixels_valid_ = false;
+    mutable std::string err_;
+};
+
+// The reference's class name, for drop-in use by the example drivers.
+using VulkanRenderer = Renderer;
+
+// Image helpers used by the drivers.
+bool writePNG(const std::string& path, const uint8_t* rgba, uint32_t width, uint32_t height);
+bool readHDR(const std::string& path, std::vector<float>& rgba, uint32_t& width, uint32_t& height);
+
+}  // namespace ohao

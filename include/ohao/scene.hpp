@@ -1,0 +1,193 @@
+// Scene model of the C++ façade: the subset of the reference's
+// Scene / Actor / Component classes that its example drivers build scenes with
+// (ohao/scene/scene.hpp, ohao/scene/actor/actor.hpp,
+// ohao/scene/component/{mesh,material,light}_component.hpp,
+// ohao/scene/asset/model.hpp:18-30, ohao/gpu/vulkan/material.hpp:11-41).
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <string_view>
+#include <typeindex>
+#include <unordered_map>
+#include <vector>
+
+#include "math.hpp"
+
+namespace ohao {
+
+struct Vertex {
+    vec3 position;
+    vec3 color{1.0f, 1.0f, 1.0f};
+    vec3 normal;
+    vec2 texCoord;
+};
+
+// Triangle mesh; the actor's MaterialComponent applies to every triangle.
+struct Model {
+    std::vector<Vertex> vertices;
+    std::vector<uint32_t> indices;
+    // Wavefront OBJ (v / vt / vn / f, polygons fanned into triangles).
+    bool loadFromOBJ(const std::string& path);
+};
+
+// Float texture registered with a Scene; referenced by Material texture ids.
+struct Texture {
+    uint32_t width = 0, height = 0, channels = 0;
+    std::vector<float> texels;   // row-major, `channels` floats per texel
+};
+
+struct Material {
+    vec3 baseColor{0.8f, 0.8f, 0.8f};
+    float roughness = 0.5f;
+    float metallic = 0.0f;
+    float specularF0 = 0.04f;      // dielectric F0 (0 = pure Lambert)
+    vec3 emissive{0.0f, 0.0f, 0.0f};
+    int32_t albedoTexture = -1;    // Scene texture ids; -1 = none
+    int32_t roughnessTexture = -1;
+    int32_t metallicTexture = -1;
+};
+
+class Transform {
+public:
+    void setPosition(vec3 p) { position_ = p; }
+    void setRotation(const quat& q) { rotation_ = q; }
+    void setScale(vec3 s) { scale_ = s; }
+    vec3 getPosition() const { return position_; }
+    quat getRotation() const { return rotation_; }
+    vec3 getScale() const { return scale_; }
+    mat4 getWorldMatrix() const { return trs(position_, rotation_, scale_); }
+
+private:
+    vec3 position_{0.0f, 0.0f, 0.0f};
+    quat rotation_{};
+    vec3 scale_{1.0f, 1.0f, 1.0f};
+};
+
+class Component {
+public:
+    virtual ~Component() = default;
+    virtual const char* getTypeName() const = 0;
+};
+
+class MeshComponent : public Component {
+public:
+    const char* getTypeName() const override { return "MeshComponent"; }
+    void setModel(std::shared_ptr<Model> m) { model_ = std::move(m); }
+    std::shared_ptr<Model> getModel() const { return model_; }
+    void setVisible(bool v) { visible_ = v; }
+    bool isVisible() const { return visible_; }
+
+private:
+    std::shared_ptr<Model> model_;
+    bool visible_ = true;
+};
+
+class MaterialComponent : public Component {
+public:
+    const char* getTypeName() const override { return "MaterialComponent"; }
+    Material& getMaterial() { return material_; }
+    const Material& getMaterial() const { return material_; }
+
+private:
+    Material material_;
+};
+
+// light_component.hpp:10-15
+enum class LightType : int { Sphere = 0, Directional = 1, Spot = 2, AreaRect = 3 };
+
+class LightComponent : public Component {
+public:
+    const char* getTypeName() const override { return "LightComponent"; }
+    void setLightType(LightType t) { type_ = t; }
+    LightType getLightType() const { return type_; }
+    void setColor(vec3 c) { color_ = c; }
+    vec3 getColor() const { return color_; }
+    void setIntensity(float i) { intensity_ = i; }
+    float getIntensity() const { return intensity_; }
+    void setRange(float r) { range_ = r; }
+    float getRange() const { return range_; }
+    void setInnerConeAngle(float a) { inner_ = a; }
+    float getInnerConeAngle() const { return inner_; }
+    void setOuterConeAngle(float a) { outer_ = a; }
+    float getOuterConeAngle() const { return outer_; }
+    void setDirection(vec3 d) { direction_ = normalize(d); }
+    vec3 getDirection() const { return direction_; }
+    void setRadius(float r) { radius_ = r; }
+    float getRadius() const { return radius_; }
+    void setAreaEdges(vec3 e1, vec3 e2) { edge1_ = e1; edge2_ = e2; }
+    vec3 getEdge1() const { return edge1_; }
+    vec3 getEdge2() const { return edge2_; }
+
+private:
+    LightType type_ = LightType::Sphere;
+    vec3 color_{1.0f, 1.0f, 1.0f};
+    float intensity_ = 1.0f, range_ = 10.0f, inner_ = 0.5f, outer_ = 0.7f, radius_ = 0.1f;
+    vec3 direction_{0.0f, -1.0f, 0.0f};
+    vec3 edge1_{1.0f, 0.0f, 0.0f}, edge2_{0.0f, 0.0f, 1.0f};
+};
+
+class Actor {
+public:
+    Actor(uint64_t id, std::string name) : id_(id), name_(std::move(name)) {}
+    uint64_t getId() const { return id_; }
+    const std::string& getName() const { return name_; }
+    void addTag(std::string_view t) { tags_.emplace_back(t); }
+    const std::vector<std::string>& getTags() const { return tags_; }
+    Transform* getTransform() { return &transform_; }
+    const Transform* getTransform() const { return &transform_; }
+
+    template <class T>
+    std::shared_ptr<T> addComponent() {
+        auto c = std::make_shared<T>();
+        components_[std::type_index(typeid(T))] = c;
+        return c;
+    }
+    template <class T>
+    std::shared_ptr<T> getComponent() const {
+        auto it = components_.find(std::type_index(typeid(T)));
+        return it == components_.end() ? nullptr : std::static_pointer_cast<T>(it->second);
+    }
+
+private:
+    uint64_t id_;
+    std::string name_;
+    std::vector<std::string> tags_;
+    Transform transform_;
+    std::unordered_map<std::type_index, std::shared_ptr<Component>> components_;
+};
+
+// component_factory.hpp:19-29 (the mesh-bearing subset)
+enum class PrimitiveType { Empty, Cube, Sphere, Platform };
+
+class Scene {
+public:
+    explicit Scene(std::string name = "Scene") : name_(std::move(name)) {}
+    const std::string& getName() const { return name_; }
+    std::shared_ptr<Actor> createActor(std::string_view name);
+    // Actor with a mesh + material for the primitive (sphere: radius 0.5,
+    // 16x16 segments, primitive_mesh_generator.hpp:12; cube: unit cube;
+    // platform: 4 x 0.2 x 4 box), default colour (0.6, 0.7, 0.8).
+    std::shared_ptr<Actor> createActorWithComponents(std::string_view name, PrimitiveType type);
+    const std::map<uint64_t, std::shared_ptr<Actor>>& getAllActors() const { return actors_; }
+    std::shared_ptr<Actor> findActor(std::string_view name) const;
+    // Textures referenced by Material::*Texture ids.
+    int32_t addTexture(Texture t);
+    Texture& getTexture(int32_t id) { return textures_.at((size_t)id); }
+    const std::vector<Texture>& getTextures() const { return textures_; }
+
+private:
+    std::string name_;
+    uint64_t next_id_ = 1;
+    std::map<uint64_t, std::shared_ptr<Actor>> actors_;
+    std::vector<Texture> textures_;
+};
+
+// Primitive meshes (primitive_mesh_generator.cpp:102-140 for the UV sphere).
+std::shared_ptr<Model> makeSphere(float radius = 0.5f, int segments = 16);
+std::shared_ptr<Model> makeBox(vec3 half_extent);
+std::shared_ptr<Model> makeQuad(vec3 a, vec3 b, vec3 c, vec3 d, vec3 normal);
+
+}  // namespace ohao

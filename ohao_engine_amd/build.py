@@ -46,6 +46,53 @@ def _compile(src: str, force: bool) -> Path:
     return obj
 
 
+FACADE = ["scene.cpp", "renderer.cpp", "image_io.cpp"]
+EXAMPLES = ["cornell_box", "turntable", "inverse_fit"]
+FACADE_LIB = PKG / "libohao.so"
+BIN = PKG / "bin"
+CXX = os.environ.get("CXX", "g++")
+CXXFLAGS = ["-O2", "-std=c++20", "-fPIC", "-I", str(INCLUDE), "-Wall", "-Wextra", "-Wno-unused-parameter"]
+ROCM_LIB = "/opt/rocm/lib"
+
+
+def _facade_deps_mtime() -> float:
+    files = list((INCLUDE / "ohao").glob("*.hpp")) + [INCLUDE / "ohpt.h", Path(__file__)]
+    return max(f.stat().st_mtime for f in files if f.exists())
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{' '.join(cmd)}\n{r.stderr[-6000:]}")
+
+
+def build_facade(force: bool = False, verbose: bool = True):
+    """C++ façade (libohao.so over libohpt.so) and the example drivers."""
+    BUILD.mkdir(exist_ok=True)
+    BIN.mkdir(exist_ok=True)
+    deps = max(_facade_deps_mtime(), LIB.stat().st_mtime)
+    objs = []
+    for src in FACADE:
+        s = PKG / "facade" / src
+        obj = BUILD / ("facade_" + Path(src).stem + ".o")
+        if force or not obj.exists() or obj.stat().st_mtime < max(s.stat().st_mtime, deps):
+            _run([CXX, *CXXFLAGS, "-c", str(s), "-o", str(obj)], f"g++ {src}")
+        objs.append(obj)
+    if force or not FACADE_LIB.exists() or FACADE_LIB.stat().st_mtime < max(o.stat().st_mtime for o in objs):
+        _run([CXX, "-shared", "-o", str(FACADE_LIB), *map(str, objs), "-L", str(PKG), "-lohpt", "-ldl",
+              "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath-link,{ROCM_LIB}"], "link libohao.so")
+    for ex in EXAMPLES:
+        s = PKG / "examples" / (ex + ".cpp")
+        if not s.exists():
+            continue
+        out = BIN / ex
+        if force or not out.exists() or out.stat().st_mtime < max(s.stat().st_mtime, FACADE_LIB.stat().st_mtime):
+            _run([CXX, *CXXFLAGS, str(s), "-o", str(out), "-L", str(PKG), "-lohao", "-lohpt",
+                  "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath-link,{ROCM_LIB}"], f"g++ {ex}")
+    if verbose:
+        print(f"built {FACADE_LIB} and {len(EXAMPLES)} example drivers in {BIN}")
+
+
 def build(force: bool = False, verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -60,6 +107,7 @@ def build(force: bool = False, verbose: bool = True) -> Path:
         os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_facade(force, verbose)
     return LIB
 
 

@@ -1,0 +1,233 @@
+"""Learned denoiser workflow on top of the C-ABI: training data rendered by the
+path tracer itself, Adam training with the library's own GPU kernels
+(ohpt_denoiser_train_step), validation against held-out high-spp renders, and
+the quality gate that falls back to plain accumulation.
+
+Replaces the reference's denoise stage (oidnDenoise,
+ohao/render/rt/denoise/oidn_denoise.hpp:14-17, selected by
+VulkanRenderer::setDenoiseMode, ohao/gpu/vulkan/renderer.hpp:182).
+
+    python -m ohao_engine_amd.denoise train --out weights.bin --report report.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from . import ohpt, scenes as S
+
+WEIGHTS = Path(__file__).resolve().parent / "weights" / "kpn_unet.bin"
+RELMSE_EPS = 1e-2
+# The network's output is used only when its relMSE against a held-out
+# high-spp render is at most this bound and below the relMSE of the plain
+# accumulation it replaces; otherwise the gate falls back to plain
+# accumulation.
+RELMSE_BOUND = 0.05
+
+
+def log(msg: str):
+    print(f"[denoise] {msg}", flush=True)
+
+
+def relmse(x, ref, eps: float = RELMSE_EPS) -> float:
+    """Rousselle-style relative MSE over pixels and RGB: mean (x-ref)^2/(ref^2+eps)."""
+    x = np.asarray(x, np.float64)[..., :3]
+    r = np.asarray(ref, np.float64)[..., :3]
+    return float(np.mean((x - r) ** 2 / (r * r + eps)))
+
+
+def relmse_torch(x, ref, eps: float = RELMSE_EPS) -> float:
+    d = x[..., :3].double() - ref[..., :3].double()
+    return float((d * d / (ref[..., :3].double() ** 2 + eps)).mean())
+
+
+def random_view(rng: np.random.Generator) -> S.Camera:
+    """A camera inside the procedural scene's extent looking at its ground."""
+    while True:
+        pos = np.array([rng.uniform(-18, 18), rng.uniform(1.5, 9.0), rng.uniform(-6, 22)])
+        tgt = np.array([rng.uniform(-15, 15), rng.uniform(0.0, 1.5), rng.uniform(-10, 8)])
+        if np.linalg.norm(tgt - pos) > 4.0:
+            return S.Camera.look_at(tuple(pos), tuple(tgt), float(rng.uniform(35.0, 60.0)))
+
+
+@dataclass
+class Sample:
+    """One view: K noisy 1-spp renders (radiance, albedo, normal, depth) and the
+    high-spp target, all device tensors (H, W, 4) / (H, W)."""
+    rad: list = field(default_factory=list)
+    alb: list = field(default_factory=list)
+    nrm: list = field(default_factory=list)
+    dep: list = field(default_factory=list)
+    target: object = None
+
+
+def render_view(ctx: ohpt.Context, cam, n_inputs: int, target_spp: int, seed: int, torch, device) -> Sample:
+    H, W = ctx.height, ctx.width
+    smp = Sample()
+    ctx.set_camera(cam)
+    for k in range(n_inputs):
+        ctx.set_render_seed(seed + k)
+        ctx.reset()
+        ctx.render(1)
+        r = torch.empty((H, W, 4), dtype=torch.float32, device=device)
+        a = torch.empty((H, W, 4), dtype=torch.float32, device=device)
+        n = torch.empty((H, W, 4), dtype=torch.float32, device=device)
+        d = torch.empty((H, W), dtype=torch.float32, device=device)
+        ctx.read_accum_device(r)
+        ctx.read_aovs_device(a, n, d)
+        smp.rad.append(r); smp.alb.append(a); smp.nrm.append(n); smp.dep.append(d)
+    ctx.set_render_seed(seed + 1_000_003)
+    ctx.reset()
+    ctx.render(target_spp)
+    t = torch.empty((H, W, 4), dtype=torch.float32, device=device)
+    ctx.read_accum_device(t)
+    smp.target = t
+    return smp
+
+
+def generate(scene_seeds, views_per_scene: int, size: int, n_inputs: int, target_spp: int, torch, device):
+    data = []
+    t0 = time.time()
+    for si, seed in enumerate(scene_seeds):
+        sc = S.procedural(width=size, height=size, seed=seed)
+        ctx = ohpt.Context(size, size)
+        ctx.load_scene(sc)
+        rng = np.random.default_rng(seed * 7919 + 1)
+        for v in range(views_per_scene):
+            data.append(render_view(ctx, random_view(rng), n_inputs, target_spp, 1 + 97 * v, torch, device))
+        ctx.close()
+        torch.cuda.synchronize()
+        log(f"scene seed {seed}: {views_per_scene} views of {size}x{size} ({n_inputs} x 1 spp + {target_spp} spp) "
+            f"[{si + 1}/{len(scene_seeds)}, {time.time() - t0:.0f}s]")
+    return data
+
+
+def make_batch(data, rng, batch: int, crop: int, torch):
+    rad, alb, nrm, dep, tgt = [], [], [], [], []
+    for _ in range(batch):
+        s = data[rng.integers(len(data))]
+        k = rng.integers(len(s.rad))
+        H, W = s.target.shape[0], s.target.shape[1]
+        y, x = int(rng.integers(0, H - crop + 1)), int(rng.integers(0, W - crop + 1))
+        fy, fx = bool(rng.integers(2)), bool(rng.integers(2))
+
+        def cut(t):
+            c = t[y:y + crop, x:x + crop]
+            if fy:
+                c = torch.flip(c, dims=[0])
+            if fx:
+                c = torch.flip(c, dims=[1])
+            return c
+
+        rad.append(cut(s.rad[k])); alb.append(cut(s.alb[k])); nrm.append(cut(s.nrm[k]))
+        dep.append(cut(s.dep[k])); tgt.append(cut(s.target))
+    st = lambda xs: torch.stack(xs).contiguous()
+    return st(rad), st(alb), st(nrm), st(dep), st(tgt)
+
+
+def train(dn: ohpt.Denoiser, data, steps: int, batch: int, crop: int, lr: float, seed: int, torch, log_every=100):
+    rng = np.random.default_rng(seed)
+    losses = []
+    t0 = time.time()
+    gpu_ms = 0.0
+    for it in range(steps):
+        r, a, n, d, t = make_batch(data, rng, batch, crop, torch)
+        # cosine decay to 10 % of the initial rate
+        lr_t = lr * (0.1 + 0.9 * 0.5 * (1.0 + math.cos(math.pi * it / max(steps - 1, 1))))
+        losses.append(dn.train_step(r, a, n, d, t, lr_t))
+        gpu_ms += dn.stats().ms_total
+        if (it + 1) % log_every == 0 or it == 0:
+            log(f"step {it + 1}/{steps}: loss {np.mean(losses[-log_every:]):.5f} lr {lr_t:.2e} "
+                f"({time.time() - t0:.0f}s, {gpu_ms / (it + 1):.2f} ms/step GPU)")
+    return losses, gpu_ms / max(steps, 1)
+
+
+@dataclass
+class GateResult:
+    enabled: bool
+    relmse_denoised: float
+    relmse_input: float
+    bound: float
+
+
+class LearnedDenoiser:
+    """Network + quality gate. `apply` returns the denoised image when the
+    gate is open and the plain accumulation otherwise."""
+
+    def __init__(self, weights: Path | str | None = WEIGHTS, device: int = 0, bound: float = RELMSE_BOUND):
+        self.dn = ohpt.Denoiser(device=device)
+        if weights is not None:
+            self.dn.load(weights)
+        self.bound = bound
+        self.gate: GateResult | None = None
+
+    def validate(self, inputs, reference) -> GateResult:
+        """inputs: (radiance, albedo, normal, depth) of a held-out noisy render;
+        reference: its high-spp render (H, W, 4)."""
+        rad, alb, nrm, dep = inputs
+        out = self.dn.infer(rad, alb, nrm, dep)
+        to_np = lambda x: x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x)
+        o = to_np(out)[0]
+        ref = to_np(reference)
+        e_d = relmse(o, ref)
+        e_in = relmse(to_np(rad), ref)
+        self.gate = GateResult(bool(e_d <= self.bound and e_d < e_in), e_d, e_in, self.bound)
+        return self.gate
+
+    def apply(self, ctx: ohpt.Context):
+        """Denoised RGBA32F of ctx's accumulation, or its plain mean when the
+        gate is closed (or was never opened by validate)."""
+        if self.gate is not None and self.gate.enabled:
+            return ctx.denoise(self.dn)
+        return ctx.read_accum()
+
+    def close(self):
+        self.dn.close()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    tr = sub.add_parser("train", help="render training data, train, save weights")
+    tr.add_argument("--out", required=True)
+    tr.add_argument("--report", default=None)
+    tr.add_argument("--scenes", type=int, default=8)
+    tr.add_argument("--views", type=int, default=8)
+    tr.add_argument("--size", type=int, default=320)
+    tr.add_argument("--inputs", type=int, default=4)
+    tr.add_argument("--target-spp", type=int, default=4096)
+    tr.add_argument("--steps", type=int, default=4000)
+    tr.add_argument("--batch", type=int, default=16)
+    tr.add_argument("--crop", type=int, default=256)
+    tr.add_argument("--lr", type=float, default=3e-4)
+    tr.add_argument("--seed", type=int, default=1)
+    tr.add_argument("--init", default=None, help="start from these weights")
+    a = ap.parse_args(argv)
+    import torch
+    dev = torch.device("cuda:0")
+    seeds = [1000 + 17 * i for i in range(a.scenes)]   # disjoint from the benchmark scene's seed 7
+    data = generate(seeds, a.views, a.size, a.inputs, a.target_spp, torch, dev)
+    dn = ohpt.Denoiser(seed=a.seed)
+    if a.init:
+        dn.load(a.init)
+    losses, ms_step = train(dn, data, a.steps, a.batch, a.crop, a.lr, a.seed, torch)
+    dn.save(a.out)
+    log(f"saved {a.out}")
+    if a.report:
+        rep = {"steps": a.steps, "batch": a.batch, "crop": a.crop, "lr": a.lr, "scenes": seeds,
+               "views_per_scene": a.views, "size": a.size, "inputs_per_view": a.inputs,
+               "target_spp": a.target_spp, "train_gpu_ms_per_step": ms_step,
+               "loss_first100": float(np.mean(losses[:100])), "loss_last100": float(np.mean(losses[-100:]))}
+        Path(a.report).write_text(json.dumps(rep, indent=1))
+    dn.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,173 @@
+// Cornell box driver over the C++ façade. Same flow as the reference's
+// examples/cornell_box.cpp:64-179: VulkanRenderer(W,H) -> initialize() ->
+// setScene -> camera -> resetAccumulation -> render() x (spp + 3) ->
+// getPixelSpan() -> PNG.
+//
+// Scene: the 0.555 m room (12 wall triangles plus a closed front wall), short
+// box rotated -17 deg, tall box rotated 15 deg, 2-triangle ceiling light with
+// emission (17, 12, 4); Lambert albedos white (0.725, 0.71, 0.68), red
+// (0.63, 0.065, 0.05), green (0.14, 0.45, 0.091). Camera at (0.278, 0.273,
+// -0.8), yaw 90, fov 39.3; depth 8, Russian roulette from bounce 3, PCG32.
+//
+//   cornell_box [out.png] [spp=64] [--size=WxH] [--seed=N] [--hdr=out.pfm] [--denoise=learned]
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "ohao/renderer.hpp"
+
+using namespace ohao;
+
+namespace {
+
+struct Tri {
+    vec3 a, b, c;
+};
+
+void quad(std::vector<Tri>& out, vec3 a, vec3 b, vec3 c, vec3 d) {
+    out.push_back({a, b, c});
+    out.push_back({a, c, d});
+}
+
+// Box of size s with its base centre at `center` minus half height, rotated
+// by `deg` about +y; faces wound outward (top included, bottom omitted).
+std::vector<Tri> box(double cx, double cy, double cz, double sx, double sy, double sz, double deg) {
+    sx *= 0.5;
+    sy *= 0.5;
+    sz *= 0.5;
+    const double th = deg * 3.141592653589793 / 180.0, c = std::cos(th), s = std::sin(th);
+    auto P = [&](double x, double y, double z) {
+        return vec3((float)(cx + c * x + s * z), (float)(cy + y), (float)(cz - s * x + c * z));
+    };
+    std::vector<Tri> t;
+    quad(t, P(-sx, -sy, sz), P(sx, -sy, sz), P(sx, sy, sz), P(-sx, sy, sz));
+    quad(t, P(sx, -sy, -sz), P(-sx, -sy, -sz), P(-sx, sy, -sz), P(sx, sy, -sz));
+    quad(t, P(sx, -sy, sz), P(sx, -sy, -sz), P(sx, sy, -sz), P(sx, sy, sz));
+    quad(t, P(-sx, -sy, -sz), P(-sx, -sy, sz), P(-sx, sy, sz), P(-sx, sy, -sz));
+    quad(t, P(-sx, sy, sz), P(sx, sy, sz), P(sx, sy, -sz), P(-sx, sy, -sz));
+    return t;
+}
+
+void add_actor(Scene& scene, const char* name, const std::vector<Tri>& tris, vec3 albedo, vec3 emission) {
+    auto a = scene.createActor(name);
+    auto m = std::make_shared<Model>();
+    for (const Tri& t : tris) {
+        const uint32_t base = (uint32_t)m->vertices.size();
+        for (vec3 p : {t.a, t.b, t.c}) {
+            Vertex v;
+            v.position = p;   // zero normal: the geometric normal is used
+            m->vertices.push_back(v);
+        }
+        m->indices.insert(m->indices.end(), {base, base + 1, base + 2});
+    }
+    auto mc = a->addComponent<MeshComponent>();
+    mc->setModel(m);
+    mc->setVisible(true);
+    Material& mat = a->addComponent<MaterialComponent>()->getMaterial();
+    mat.baseColor = albedo;
+    mat.roughness = 1.0f;
+    mat.metallic = 0.0f;
+    mat.specularF0 = 0.0f;
+    mat.emissive = emission;
+}
+
+bool write_pfm(const std::string& path, const std::vector<float>& rgba, uint32_t w, uint32_t h) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return false;
+    f << "PF\n" << w << " " << h << "\n-1.0\n";
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x)
+            f.write(reinterpret_cast<const char*>(&rgba[((size_t)(h - 1 - y) * w + x) * 4]), 12);
+    return (bool)f;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string out = "cornell_box.png", hdr;
+    int spp = 64;
+    uint32_t W = 256, H = 256, seed = 1;
+    DenoiseMode dmode = DenoiseMode::None;
+    int pos = 0;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        if (a.rfind("--size=", 0) == 0) std::sscanf(a.c_str() + 7, "%ux%u", &W, &H);
+        else if (a.rfind("--seed=", 0) == 0) seed = (uint32_t)std::strtoul(a.c_str() + 7, nullptr, 10);
+        else if (a.rfind("--hdr=", 0) == 0) hdr = a.substr(6);
+        else if (a.rfind("--denoise=", 0) == 0) dmode = parseDenoiseMode(a.substr(10));
+        else if (pos == 0) { out = a; ++pos; }
+        else if (pos == 1) { spp = std::max(1, std::atoi(a.c_str())); ++pos; }
+    }
+    std::printf("ohao cornell_box: %ux%u @ %d spp\n", W, H, spp);
+
+    VulkanRenderer renderer(W, H);
+    if (!renderer.initialize()) {
+        std::fprintf(stderr, "FATAL: renderer init failed: %s\n", renderer.lastError().c_str());
+        return 1;
+    }
+    Scene scene("Cornell Box");
+    const double R = 0.555, zf = -0.81;
+    auto P = [](double x, double y, double z) { return vec3((float)x, (float)y, (float)z); };
+    const vec3 white(0.725f, 0.71f, 0.68f), red(0.63f, 0.065f, 0.05f), green(0.14f, 0.45f, 0.091f), none(0.0f);
+    std::vector<Tri> t;
+    quad(t, P(0, 0, zf), P(0, 0, R), P(R, 0, R), P(R, 0, zf));
+    add_actor(scene, "Floor", t, white, none);
+    t.clear(); quad(t, P(0, R, zf), P(R, R, zf), P(R, R, R), P(0, R, R));
+    add_actor(scene, "Ceiling", t, white, none);
+    t.clear(); quad(t, P(0, 0, R), P(0, R, R), P(R, R, R), P(R, 0, R));
+    add_actor(scene, "Back", t, white, none);
+    t.clear(); quad(t, P(R, 0, zf), P(R, 0, R), P(R, R, R), P(R, R, zf));
+    add_actor(scene, "Left", t, red, none);
+    t.clear(); quad(t, P(0, 0, zf), P(0, R, zf), P(0, R, R), P(0, 0, R));
+    add_actor(scene, "Right", t, green, none);
+    t.clear(); quad(t, P(0, 0, zf), P(R, 0, zf), P(R, R, zf), P(0, R, zf));
+    add_actor(scene, "Front", t, white, none);
+    add_actor(scene, "ShortBox", box(0.1855, 0.0825, 0.169, 0.165, 0.165, 0.165, -17.0), white, none);
+    add_actor(scene, "TallBox", box(0.3685, 0.165, 0.3513, 0.165, 0.33, 0.165, 15.0), white, none);
+    const double ly = R - 1e-3;
+    t.clear(); quad(t, P(0.213, ly, 0.227), P(0.343, ly, 0.227), P(0.343, ly, 0.332), P(0.213, ly, 0.332));
+    add_actor(scene, "Light", t, vec3(0.78f), vec3(17.0f, 12.0f, 4.0f));
+
+    renderer.setScene(&scene);
+    RTRenderSettings st;
+    st.maxBounces = 8;
+    st.rrStart = 3;
+    st.sobol = false;
+    renderer.setRTRenderSettings(st);
+    auto& cam = renderer.getCamera();
+    cam.setPosition({0.278f, 0.273f, -0.8f});
+    cam.setRotation(0.0f, 90.0f);
+    cam.setFov(39.3f);
+    renderer.setRenderMode(RenderMode::RTOffline);
+    if (dmode != DenoiseMode::None) renderer.setDenoiseMode(dmode);
+    std::printf("Denoise mode: %s\n", denoiseModeName(renderer.getDenoiseMode()));
+    renderer.setRenderSeed(seed);
+    renderer.resetAccumulation();
+
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < spp + 3; ++i) renderer.render();
+    const auto px = renderer.getPixelSpan();
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (!renderer.lastError().empty()) std::fprintf(stderr, "warning: %s\n", renderer.lastError().c_str());
+    std::printf("Done: %.1f ms, %u samples per pixel\n", ms, renderer.getSampleCount());
+    if (px.empty() || !writePNG(out, px.data(), W, H)) {
+        std::fprintf(stderr, "FATAL: could not write %s\n", out.c_str());
+        return 1;
+    }
+    std::printf("Saved: %s\n", out.c_str());
+    if (!hdr.empty()) {
+        std::vector<float> beauty, albedo, normal;
+        uint32_t w = 0, h = 0;
+        if (!renderer.readbackHDRBuffers(beauty, albedo, normal, w, h) || !write_pfm(hdr, beauty, w, h)) {
+            std::fprintf(stderr, "FATAL: could not write %s\n", hdr.c_str());
+            return 1;
+        }
+        std::printf("Saved HDR: %s\n", hdr.c_str());
+    }
+    return 0;
+}

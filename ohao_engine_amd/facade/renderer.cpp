@@ -1,0 +1,485 @@
+// C++ façade over the ohpt C-ABI (include/ohao/renderer.hpp).
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+
+#include "ohao/renderer.hpp"
+
+namespace ohao {
+
+DenoiseMode parseDenoiseMode(std::string_view s) {
+    std::string l(s);
+    std::transform(l.begin(), l.end(), l.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+    if (l == "oidn") return DenoiseMode::OIDN;
+    if (l == "nrd") return DenoiseMode::NRD;
+    if (l == "atrous" || l == "svgf") return DenoiseMode::Atrous;
+    if (l == "dlssrr" || l == "dlss-rr" || l == "dlss_rr") return DenoiseMode::DLSSRR;
+    if (l == "learned" || l == "kpn") return DenoiseMode::Learned;
+    return DenoiseMode::None;
+}
+
+const char* denoiseModeName(DenoiseMode m) {
+    switch (m) {
+        case DenoiseMode::None: return "None";
+        case DenoiseMode::OIDN: return "OIDN";
+        case DenoiseMode::NRD: return "NRD";
+        case DenoiseMode::Atrous: return "Atrous";
+        case DenoiseMode::DLSSRR: return "DLSSRR";
+        case DenoiseMode::Learned: return "Learned";
+    }
+    return "?";
+}
+
+// Camera::updateVectors (camera.cpp:25-40), evaluated in double precision.
+void Camera::update() {
+    const double deg = 3.141592653589793 / 180.0;
+    const double p = std::clamp((double)pitch_, -89.0, 89.0) * deg, y = (double)yaw_ * deg;
+    double f[3] = {std::cos(y) * std::cos(p), std::sin(p), std::sin(y) * std::cos(p)};
+    auto norm = [](double* v) {
+        const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        v[0] /= l; v[1] /= l; v[2] /= l;
+    };
+    norm(f);
+    double r[3] = {-f[2], 0.0, f[0]};   // f x (0, 1, 0)
+    norm(r);
+    double u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
+    norm(u);
+    front_ = vec3((float)f[0], (float)f[1], (float)f[2]);
+    right_ = vec3((float)r[0], (float)r[1], (float)r[2]);
+    up_ = vec3((float)u[0], (float)u[1], (float)u[2]);
+}
+
+ohpt_camera Camera::toOhpt() const {
+    ohpt_camera c{};
+    const vec3 v[4] = {position_, front_, right_, up_};
+    float* dst[4] = {c.position, c.forward, c.right, c.up};
+    for (int i = 0; i < 4; ++i) {
+        dst[i][0] = v[i].x;
+        dst[i][1] = v[i].y;
+        dst[i][2] = v[i].z;
+    }
+    c.tan_half_fov_y = (float)std::tan((double)fov_ * (3.141592653589793 / 180.0) * 0.5);
+    return c;
+}
+
+namespace {
+
+// (ACES(0.5 x))^(1/2.2) -> 8 bit, the readback transform of the RGBA8 image.
+uint8_t tonemap8(float x) {
+    const float u = 0.5f * std::max(x, 0.0f);
+    float a = (u * (2.51f * u + 0.03f)) / (u * (2.43f * u + 0.59f) + 0.14f);
+    a = std::clamp(a, 0.0f, 1.0f);
+    const float t = std::pow(a, 1.0f / 2.2f);
+    return (uint8_t)std::min(255.0f, std::max(0.0f, t * 255.0f + 0.5f));
+}
+
+std::string default_weights() {
+    Dl_info info{};
+    if (dladdr(reinterpret_cast<void*>(&ohpt_create), &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        const size_t s = p.find_last_of('/');
+        return (s == std::string::npos ? std::string(".") : p.substr(0, s)) + "/weights/kpn_unet.bin";
+    }
+    return "weights/kpn_unet.bin";
+}
+
+double relmse(const std::vector<float>& x, const std::vector<float>& ref) {
+    double s = 0.0;
+    const size_t n = ref.size() / 4;
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const double r = ref[i * 4 + k], d = x[i * 4 + k] - r;
+            s += d * d / (r * r + 1e-2);
+        }
+    return s / (3.0 * (double)std::max<size_t>(n, 1));
+}
+
+}  // namespace
+
+Renderer::Renderer(uint32_t width, uint32_t height, int device) : W_(width), H_(height), device_(device) {}
+
+Renderer::~Renderer() { shutdown(); }
+
+bool Renderer::fail(const std::string& what) const {
+    err_ = what + ": " + ohpt_last_error();
+    return false;
+}
+
+bool Renderer::initialize() {
+    if (ctx_) return true;
+    if (ohpt_create(&ctx_, device_, W_, H_) != OHPT_OK) return fail("ohpt_create");
+    setRTRenderSettings(settings_);
+    return true;
+}
+
+void Renderer::shutdown() {
+    if (dn_) {
+        ohpt_denoiser_destroy(dn_);
+        dn_ = nullptr;
+    }
+    if (ctx_) {
+        ohpt_destroy(ctx_);
+        ctx_ = nullptr;
+    }
+}
+
+void Renderer::setRTRenderSettings(const RTRenderSettings& s) {
+    settings_ = s;
+    if (!ctx_) return;
+    ohpt_settings st{};
+    st.max_bounces = s.maxBounces;
+    st.rr_start = s.rrStart;
+    st.sampler = s.sobol ? OHPT_SAMPLER_SOBOL_OWEN : OHPT_SAMPLER_PCG32;
+    st.flags = OHPT_FLAG_AOVS | (s.fireflyClamp ? OHPT_FLAG_FIREFLY_CLAMP : 0u);
+    st.firefly_clamp = s.fireflyClampValue;
+    if (ohpt_set_settings(ctx_, &st) != OHPT_OK) fail("ohpt_set_settings");
+}
+
+void Renderer::setScene(Scene* scene) {
+    scene_ = scene;
+    scene_dirty_ = true;
+    resetAccumulation();
+}
+
+bool Renderer::ensure_scene() const {
+    if (!scene_dirty_ && !env_dirty_) return true;
+    return const_cast<Renderer*>(this)->updateSceneBuffers();
+}
+
+bool Renderer::updateSceneBuffers() {
+    if (!ctx_ && !initialize()) return false;
+    if (scene_ && scene_dirty_) {
+        if (ohpt_scene_begin(ctx_) != OHPT_OK) return fail("ohpt_scene_begin");
+        actor_mat_.clear();
+        uint32_t slot = 0;
+        for (const auto& [id, a] : scene_->getAllActors()) {
+            auto mc = a->getComponent<MeshComponent>();
+            if (!mc || !mc->isVisible() || !mc->getModel()) continue;
+            const Model& m = *mc->getModel();
+            if (m.indices.empty()) continue;
+            const size_t nv = m.vertices.size();
+            std::vector<float> pos(nv * 3), nrm(nv * 3), uv(nv * 2);
+            for (size_t i = 0; i < nv; ++i) {
+                const Vertex& v = m.vertices[i];
+                pos[i * 3] = v.position.x; pos[i * 3 + 1] = v.position.y; pos[i * 3 + 2] = v.position.z;
+                nrm[i * 3] = v.normal.x;   nrm[i * 3 + 1] = v.normal.y;   nrm[i * 3 + 2] = v.normal.z;
+                uv[i * 2] = v.texCoord.x;  uv[i * 2 + 1] = v.texCoord.y;
+            }
+            // all-zero vertex normals: shade with the geometric normal
+            const bool has_nrm = std::any_of(nrm.begin(), nrm.end(), [](float v) { return v != 0.0f; });
+            const mat4 world = a->getTransform()->getWorldMatrix();
+            if (ohpt_scene_add_mesh(ctx_, pos.data(), has_nrm ? nrm.data() : nullptr, uv.data(), (uint32_t)nv,
+                                    m.indices.data(),
+                                    (uint32_t)(m.indices.size() / 3), nullptr, slot, world.data()) != OHPT_OK)
+                return fail("ohpt_scene_add_mesh");
+            actor_mat_.push_back(slot++);
+        }
+        if (ohpt_scene_commit(ctx_) != OHPT_OK) return fail("ohpt_scene_commit");
+        tex_ids_.clear();
+        for (const Texture& t : scene_->getTextures()) {
+            int32_t id = -1;
+            if (ohpt_upload_texture(ctx_, &id, t.texels.data(), t.width, t.height, t.channels) != OHPT_OK)
+                return fail("ohpt_upload_texture");
+            tex_ids_.push_back(id);
+        }
+        if (!upload_materials() || !upload_lights()) return false;
+        scene_dirty_ = false;
+    }
+    if (env_dirty_) {
+        if (ohpt_upload_env(ctx_, env_pixels_.empty() ? nullptr : env_pixels_.data(), env_w_, env_h_) != OHPT_OK)
+            return fail("ohpt_upload_env");
+        if (ohpt_set_env_intensity(ctx_, env_scale_) != OHPT_OK) return fail("ohpt_set_env_intensity");
+        env_dirty_ = false;
+    }
+    return true;
+}
+
+bool Renderer::upload_materials() {
+    std::vector<ohpt_material> mats;
+    auto tex = [&](int32_t id) { return (id >= 0 && (size_t)id < tex_ids_.size()) ? tex_ids_[(size_t)id] : -1; };
+    for (const auto& [id, a] : scene_->getAllActors()) {
+        auto mc = a->getComponent<MeshComponent>();
+        if (!mc || !mc->isVisible() || !mc->getModel() || mc->getModel()->indices.empty()) continue;
+        auto matc = a->getComponent<MaterialComponent>();
+        const Material m = matc ? matc->getMaterial() : Material{};
+        ohpt_material o{};
+        o.base_color[0] = m.baseColor.x; o.base_color[1] = m.baseColor.y; o.base_color[2] = m.baseColor.z;
+        o.roughness = m.roughness;
+        o.metallic = m.metallic;
+        o.specular_f0 = m.specularF0;
+        o.emissive[0] = m.emissive.x; o.emissive[1] = m.emissive.y; o.emissive[2] = m.emissive.z;
+        o.albedo_tex = tex(m.albedoTexture);
+        o.roughness_tex = tex(m.roughnessTexture);
+        o.metallic_tex = tex(m.metallicTexture);
+        mats.push_back(o);
+    }
+    if (mats.empty()) mats.push_back(ohpt_material{});
+    if (ohpt_upload_materials(ctx_, mats.data(), (uint32_t)mats.size()) != OHPT_OK)
+        return fail("ohpt_upload_materials");
+    return true;
+}
+
+// GPULight packing of light_upload.cpp:157-181.
+bool Renderer::upload_lights() {
+    std::vector<ohpt_light> ls;
+    for (const auto& [id, a] : scene_->getAllActors()) {
+        auto lc = a->getComponent<LightComponent>();
+        if (!lc) continue;
+        ohpt_light g{};
+        const vec3 p = a->getTransform()->getPosition(), c = lc->getColor(), d = lc->getDirection();
+        g.position_type[0] = p.x; g.position_type[1] = p.y; g.position_type[2] = p.z;
+        g.position_type[3] = (float)(int)lc->getLightType();
+        g.color_intensity[0] = c.x; g.color_intensity[1] = c.y; g.color_intensity[2] = c.z;
+        g.color_intensity[3] = lc->getIntensity();
+        g.dir_param[0] = d.x; g.dir_param[1] = d.y; g.dir_param[2] = d.z;
+        g.dir_param[3] = lc->getLightType() == LightType::Spot ? lc->getInnerConeAngle() : lc->getRadius();
+        if (lc->getLightType() == LightType::AreaRect) {
+            const vec3 e1 = lc->getEdge1(), e2 = lc->getEdge2();
+            g.extra[0] = e1.x; g.extra[1] = e1.y; g.extra[2] = e1.z;
+            g.extra2[0] = e2.x; g.extra2[1] = e2.y; g.extra2[2] = e2.z;
+            g.extra2[3] = length(cross(e1, e2));
+        } else {
+            g.extra[3] = lc->getOuterConeAngle();
+        }
+        ls.push_back(g);
+    }
+    if (ohpt_upload_lights(ctx_, ls.empty() ? nullptr : ls.data(), (uint32_t)ls.size()) != OHPT_OK)
+        return fail("ohpt_upload_lights");
+    return true;
+}
+
+bool Renderer::updateRTMaterialParams() {
+    if (!ctx_ || !scene_ || scene_dirty_) return false;
+    if (!upload_materials()) return false;
+    resetAccumulation();
+    return true;
+}
+
+bool Renderer::updateRTLightParams() {
+    if (!ctx_ || !scene_ || scene_dirty_) return false;
+    if (!upload_lights()) return false;
+    resetAccumulation();
+    return true;
+}
+
+bool Renderer::updateTexture(int32_t id) {
+    if (!ctx_ || !scene_ || id < 0 || (size_t)id >= tex_ids_.size()) return false;
+    const Texture& t = scene_->getTexture(id);
+    if (ohpt_update_texture(ctx_, tex_ids_[(size_t)id], t.texels.data(), 0) != OHPT_OK)
+        return fail("ohpt_update_texture");
+    resetAccumulation();
+    return true;
+}
+
+void Renderer::setEnvironmentMap(std::string_view path) {
+    env_path_ = std::string(path);
+    if (env_path_.empty()) {
+        env_pixels_.clear();
+        env_w_ = env_h_ = 0;
+        env_dirty_ = true;
+        return;
+    }
+    if (env_path_ == env_loaded_) return;
+    std::vector<float> px;
+    uint32_t w = 0, h = 0;
+    if (!readHDR(env_path_, px, w, h)) {
+        err_ = "setEnvironmentMap: cannot read " + env_path_;
+        return;
+    }
+    setEnvironmentPixels(std::move(px), w, h);
+    env_loaded_ = env_path_;
+}
+
+void Renderer::setEnvironmentPixels(std::vector<float> rgba, uint32_t width, uint32_t height) {
+    env_pixels_ = std::move(rgba);
+    env_w_ = width;
+    env_h_ = height;
+    env_dirty_ = true;
+    resetAccumulation();
+}
+
+void Renderer::setEnvIntensityScale(float s) {
+    env_scale_ = std::max(0.0f, s);
+    if (ctx_ && !env_dirty_) ohpt_set_env_intensity(ctx_, env_scale_);
+    resetAccumulation();
+}
+
+void Renderer::resetAccumulation() {
+    samples_ = 0;
+    since_reset_ = 0;
+    pixels_valid_ = false;
+    if (ctx_) ohpt_reset_accumulation(ctx_);
+}
+
+void Renderer::setRenderSeed(uint32_t seed) {
+    seed_ = seed;
+    if (ctx_) ohpt_set_render_seed(ctx_, seed);
+}
+
+bool Renderer::renderSamples(uint32_t spp) {
+    if (!ctx_ && !initialize()) return false;
+    if (!ensure_scene()) return false;
+    const ohpt_camera cam = camera_.toOhpt();
+    if (ohpt_set_camera(ctx_, &cam) != OHPT_OK) return fail("ohpt_set_camera");
+    if (spp == 0) return true;
+    if (ohpt_render(ctx_, spp, 0) != OHPT_OK) return fail("ohpt_render");
+    samples_ += spp;
+    pixels_valid_ = false;
+    return true;
+}
+
+void Renderer::render() {
+    if (mode_ != RenderMode::RTOffline && mode_ != RenderMode::RTRealtime) {
+        err_ = "render: only the path-traced modes are provided";
+        return;
+    }
+    if (since_reset_ < latency_) {
+        ++since_reset_;
+        renderSamples(0);
+        return;
+    }
+    renderSamples(1);
+}
+
+void Renderer::resize(uint32_t width, uint32_t height) {
+    W_ = width;
+    H_ = height;
+    pixels_.clear();
+    pixels_valid_ = false;
+    if (ctx_ && ohpt_resize(ctx_, width, height) != OHPT_OK) fail("ohpt_resize");
+    resetAccumulation();
+}
+
+const uint8_t* Renderer::getPixels() const {
+    if (!ctx_) return nullptr;
+    if (pixels_valid_) return pixels_.data();
+    const size_t n = (size_t)W_ * H_;
+    pixels_.resize(n * 4);
+    const bool learned = (denoise_ == DenoiseMode::Learned || denoise_ == DenoiseMode::OIDN);
+    if (learned && dn_ && gate_open_ && samples_ > 0) {
+        std::vector<float> den(n * 4);
+        if (ohpt_denoise(ctx_, dn_, den.data(), 0) != OHPT_OK) {
+            fail("ohpt_denoise");
+            return nullptr;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            for (int k = 0; k < 3; ++k) pixels_[i * 4 + k] = tonemap8(den[i * 4 + k]);
+            pixels_[i * 4 + 3] = 255;
+        }
+    } else if (ohpt_read_pixels(ctx_, pixels_.data(), 0) != OHPT_OK) {
+        fail("ohpt_read_pixels");
+        return nullptr;
+    }
+    pixels_valid_ = true;
+    return pixels_.data();
+}
+
+bool Renderer::readbackHDRBuffers(std::vector<float>& beauty, std::vector<float>& albedo, std::vector<float>& normal,
+                                  uint32_t& w, uint32_t& h) {
+    if (!ctx_) return false;
+    const size_t n = (size_t)W_ * H_;
+    beauty.resize(n * 4);
+    albedo.resize(n * 4);
+    normal.resize(n * 4);
+    if (ohpt_read_accum(ctx_, beauty.data(), 0) != OHPT_OK) return fail("ohpt_read_accum");
+    if (ohpt_read_aovs(ctx_, albedo.data(), normal.data(), nullptr, 0) != OHPT_OK) return fail("ohpt_read_aovs");
+    w = W_;
+    h = H_;
+    return true;
+}
+
+bool Renderer::ensure_denoiser() {
+    if (dn_) return true;
+    if (!ctx_ && !initialize()) return false;
+    if (ohpt_denoiser_create(&dn_, device_, 1) != OHPT_OK) return fail("ohpt_denoiser_create");
+    return loadDenoiserWeights(dn_weights_.empty() ? default_weights() : dn_weights_);
+}
+
+bool Renderer::loadDenoiserWeights(const std::string& path) {
+    dn_weights_ = path;
+    if (!dn_) return true;   // loaded when the denoiser is created
+    std::ifstream f(path, std::ios::binary);
+    uint32_t hdr[3] = {0, 0, 0};
+    uint64_t n = 0;
+    ohpt_denoiser_param_count(dn_, &n);
+    if (!f || !f.read(reinterpret_cast<char*>(hdr), 12) || hdr[0] != 0x4F48444Eu || hdr[1] != 1u || hdr[2] != n) {
+        err_ = "loadDenoiserWeights: " + path + " is not a parameter file for this network";
+        return false;
+    }
+    std::vector<float> p(n);
+    if (!f.read(reinterpret_cast<char*>(p.data()), (std::streamsize)(n * 4))) {
+        err_ = "loadDenoiserWeights: short file " + path;
+        return false;
+    }
+    if (ohpt_denoiser_set_params(dn_, p.data(), n) != OHPT_OK) return fail("ohpt_denoiser_set_params");
+    pixels_valid_ = false;
+    return true;
+}
+
+void Renderer::setDenoiseMode(DenoiseMode mode) {
+    pixels_valid_ = false;
+    if (mode == DenoiseMode::NRD || mode == DenoiseMode::Atrous || mode == DenoiseMode::DLSSRR) {
+        err_ = std::string("setDenoiseMode: ") + denoiseModeName(mode) + " is not provided; using None";
+        denoise_ = DenoiseMode::None;
+        return;
+    }
+    denoise_ = mode;
+    if (mode != DenoiseMode::None && !ensure_denoiser()) denoise_ = DenoiseMode::None;
+}
+
+bool Renderer::validateDenoiser(uint32_t input_spp, uint32_t reference_spp, float bound, float* relmse_out) {
+    if (!ensure_denoiser() || !ensure_scene()) return false;
+    const size_t n = (size_t)W_ * H_;
+    const uint32_t seed = seed_;
+    std::vector<float> ref(n * 4), noisy(n * 4), den(n * 4);
+    ohpt_set_render_seed(ctx_, seed + 0x40000000u);
+    resetAccumulation();
+    if (!renderSamples(reference_spp) || ohpt_read_accum(ctx_, ref.data(), 0) != OHPT_OK) return false;
+    ohpt_set_render_seed(ctx_, seed);
+    resetAccumulation();
+    if (!renderSamples(input_spp) || ohpt_read_accum(ctx_, noisy.data(), 0) != OHPT_OK) return false;
+    if (ohpt_denoise(ctx_, dn_, den.data(), 0) != OHPT_OK) return fail("ohpt_denoise");
+    const double e = relmse(den, ref), e_plain = relmse(noisy, ref);
+    gate_open_ = e <= bound && e < e_plain;
+    if (relmse_out) *relmse_out = (float)e;
+    resetAccumulation();
+    return gate_open_;
+}
+
+bool Renderer::adjoint(const std::vector<float>& dl_dimage, uint32_t spp, bool deterministic) {
+    if (!ctx_ || dl_dimage.size() != (size_t)W_ * H_ * 4) {
+        err_ = "adjoint: dl_dimage must hold width*height RGBA floats";
+        return false;
+    }
+    if (!ensure_scene()) return false;
+    const ohpt_camera cam = camera_.toOhpt();
+    if (ohpt_set_camera(ctx_, &cam) != OHPT_OK) return fail("ohpt_set_camera");
+    if (ohpt_adjoint(ctx_, dl_dimage.data(), 0, spp, deterministic ? OHPT_ADJOINT_DETERMINISTIC : 0u) != OHPT_OK)
+        return fail("ohpt_adjoint");
+    return true;
+}
+
+bool Renderer::zeroGradients() {
+    if (!ctx_) return false;
+    return ohpt_zero_gradients(ctx_) == OHPT_OK || fail("ohpt_zero_gradients");
+}
+
+bool Renderer::textureGradient(int32_t id, std::vector<float>& out) {
+    if (!ctx_ || !scene_ || id < 0 || (size_t)id >= tex_ids_.size()) return false;
+    const Texture& t = scene_->getTexture(id);
+    out.resize((size_t)t.width * t.height * t.channels);
+    if (ohpt_read_texture_gradient(ctx_, tex_ids_[(size_t)id], out.data(), 0) != OHPT_OK)
+        return fail("ohpt_read_texture_gradient");
+    return true;
+}
+
+ohpt_stats Renderer::lastStats() const {
+    ohpt_stats s{};
+    if (ctx_) ohpt_get_stats(ctx_, &s);
+    return s;
+}
+
+}  // namespace ohao

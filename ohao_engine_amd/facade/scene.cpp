@@ -1,0 +1,189 @@
+// Scene model of the C++ façade (include/ohao/scene.hpp).
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <tuple>
+
+#include "ohao/scene.hpp"
+
+namespace ohao {
+
+std::shared_ptr<Actor> Scene::createActor(std::string_view name) {
+    auto a = std::make_shared<Actor>(next_id_, std::string(name));
+    actors_[next_id_++] = a;
+    return a;
+}
+
+std::shared_ptr<Actor> Scene::createActorWithComponents(std::string_view name, PrimitiveType type) {
+    auto a = createActor(name);
+    std::shared_ptr<Model> m;
+    switch (type) {
+        case PrimitiveType::Sphere: m = makeSphere(0.5f, 16); break;
+        case PrimitiveType::Cube: m = makeBox(vec3(0.5f)); break;
+        case PrimitiveType::Platform: m = makeBox(vec3(2.0f, 0.1f, 2.0f)); break;
+        case PrimitiveType::Empty: return a;
+    }
+    auto mc = a->addComponent<MeshComponent>();
+    mc->setModel(m);
+    mc->setVisible(true);
+    auto mat = a->addComponent<MaterialComponent>();
+    mat->getMaterial().baseColor = vec3(0.6f, 0.7f, 0.8f);   // component_factory.cpp:101
+    return a;
+}
+
+std::shared_ptr<Actor> Scene::findActor(std::string_view name) const {
+    for (const auto& [id, a] : actors_)
+        if (a->getName() == name) return a;
+    return nullptr;
+}
+
+int32_t Scene::addTexture(Texture t) {
+    textures_.push_back(std::move(t));
+    return (int32_t)textures_.size() - 1;
+}
+
+// UV sphere: (segments+1)^2 vertices on a latitude/longitude grid, two
+// triangles per cell (the layout of primitive_mesh_generator.cpp:102-140).
+std::shared_ptr<Model> makeSphere(float radius, int segments) {
+    auto m = std::make_shared<Model>();
+    const float pi = 3.14159265358979f;
+    for (int y = 0; y <= segments; ++y) {
+        const float v = (float)y / segments, phi = v * pi;
+        for (int x = 0; x <= segments; ++x) {
+            const float u = (float)x / segments, th = u * 2.0f * pi;
+            Vertex vt;
+            vt.position = vec3(radius * std::sin(phi) * std::cos(th), radius * std::cos(phi),
+                               radius * std::sin(phi) * std::sin(th));
+            vt.normal = normalize(vt.position);
+            vt.texCoord = vec2(u, v);
+            m->vertices.push_back(vt);
+        }
+    }
+    for (int y = 0; y < segments; ++y)
+        for (int x = 0; x < segments; ++x) {
+            const uint32_t c = (uint32_t)(y * (segments + 1) + x), r = c + 1, b = c + (uint32_t)(segments + 1),
+                           br = b + 1;
+            m->indices.insert(m->indices.end(), {c, b, r, r, b, br});
+        }
+    return m;
+}
+
+std::shared_ptr<Model> makeQuad(vec3 a, vec3 b, vec3 c, vec3 d, vec3 normal) {
+    auto m = std::make_shared<Model>();
+    const vec2 uv[4] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+    const vec3 p[4] = {a, b, c, d};
+    for (int i = 0; i < 4; ++i) {
+        Vertex v;
+        v.position = p[i];
+        v.normal = normal;
+        v.texCoord = uv[i];
+        m->vertices.push_back(v);
+    }
+    m->indices = {0, 1, 2, 0, 2, 3};
+    return m;
+}
+
+std::shared_ptr<Model> makeBox(vec3 h) {
+    auto m = std::make_shared<Model>();
+    const vec3 n[6] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (const vec3& f : n) {
+        // two tangent directions of the face, right-handed with the normal
+        const vec3 t = std::fabs(f.y) > 0.5f ? vec3(1, 0, 0) : vec3(0, 1, 0);
+        const vec3 s = cross(t, f);
+        const vec3 c = f * h;
+        const vec3 e1 = t * h, e2 = s * h;
+        const vec3 q[4] = {c - e1 - e2, c + e1 - e2, c + e1 + e2, c - e1 + e2};
+        const uint32_t base = (uint32_t)m->vertices.size();
+        const vec2 uv[4] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+        for (int i = 0; i < 4; ++i) {
+            Vertex v;
+            v.position = q[i];
+            v.normal = f;
+            v.texCoord = uv[i];
+            m->vertices.push_back(v);
+        }
+        // wind counter-clockwise seen from outside
+        if (dot(cross(q[1] - q[0], q[2] - q[0]), f) > 0)
+            m->indices.insert(m->indices.end(), {base, base + 1, base + 2, base, base + 2, base + 3});
+        else
+            m->indices.insert(m->indices.end(), {base, base + 2, base + 1, base, base + 3, base + 2});
+    }
+    return m;
+}
+
+bool Model::loadFromOBJ(const std::string& path) {
+    std::ifstream f(path);
+    if (!f) return false;
+    std::vector<vec3> P, N;
+    std::vector<vec2> T;
+    std::map<std::tuple<int, int, int>, uint32_t> index_of;
+    vertices.clear();
+    indices.clear();
+    std::string line;
+    auto resolve = [](int i, size_t n) { return i < 0 ? (int)n + i : i - 1; };
+    while (std::getline(f, line)) {
+        std::istringstream ss(line);
+        std::string tag;
+        ss >> tag;
+        if (tag == "v") {
+            vec3 p;
+            ss >> p.x >> p.y >> p.z;
+            P.push_back(p);
+        } else if (tag == "vn") {
+            vec3 n;
+            ss >> n.x >> n.y >> n.z;
+            N.push_back(n);
+        } else if (tag == "vt") {
+            vec2 t;
+            ss >> t.x >> t.y;
+            T.push_back(t);
+        } else if (tag == "f") {
+            std::vector<uint32_t> poly;
+            std::string tok;
+            while (ss >> tok) {
+                int vi = 0, ti = 0, ni = 0;
+                if (std::sscanf(tok.c_str(), "%d/%d/%d", &vi, &ti, &ni) == 3) {
+                } else if (std::sscanf(tok.c_str(), "%d//%d", &vi, &ni) == 2) {
+                    ti = 0;
+                } else if (std::sscanf(tok.c_str(), "%d/%d", &vi, &ti) == 2) {
+                    ni = 0;
+                } else if (std::sscanf(tok.c_str(), "%d", &vi) == 1) {
+                    ti = ni = 0;
+                } else {
+                    return false;
+                }
+                const int v = resolve(vi, P.size());
+                const int t = ti ? resolve(ti, T.size()) : -1;
+                const int n = ni ? resolve(ni, N.size()) : -1;
+                if (v < 0 || v >= (int)P.size() || t >= (int)T.size() || n >= (int)N.size()) return false;
+                const auto key = std::make_tuple(v, t, n);
+                auto it = index_of.find(key);
+                if (it == index_of.end()) {
+                    Vertex vt;
+                    vt.position = P[(size_t)v];
+                    if (t >= 0) vt.texCoord = T[(size_t)t];
+                    if (n >= 0) vt.normal = N[(size_t)n];
+                    it = index_of.emplace(key, (uint32_t)vertices.size()).first;
+                    vertices.push_back(vt);
+                }
+                poly.push_back(it->second);
+            }
+            for (size_t k = 1; k + 1 < poly.size(); ++k) indices.insert(indices.end(), {poly[0], poly[k], poly[k + 1]});
+        }
+    }
+    // vertices without a normal get the area-weighted face normal
+    std::vector<vec3> acc(vertices.size(), vec3(0.0f));
+    for (size_t i = 0; i + 2 < indices.size(); i += 3) {
+        const vec3 a = vertices[indices[i]].position, b = vertices[indices[i + 1]].position,
+                   c = vertices[indices[i + 2]].position;
+        const vec3 fn = cross(b - a, c - a);
+        for (int k = 0; k < 3; ++k) acc[indices[i + k]] = acc[indices[i + k]] + fn;
+    }
+    for (size_t i = 0; i < vertices.size(); ++i)
+        if (length(vertices[i].normal) == 0.0f && length(acc[i]) > 0.0f) vertices[i].normal = normalize(acc[i]);
+    return !indices.empty();
+}
+
+}  // namespace ohao
