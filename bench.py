@@ -87,8 +87,8 @@ def denoise_leg(ctx, sc, ref_spp, save_dir=None):
     """GPU time to equal relMSE: the learned denoiser on a 1-spp render against
     plain progressive accumulation, both scored against a held-out high-spp
     render of the benchmark view (a scene seed never used for training). The
-    reference is two independent halves, whose disagreement estimates its own
-    noise floor."""
+    reference is the mean of two independent halves; their disagreement is
+    reported as a measure of the reference's own noise."""
     from ohao_engine_amd import ohpt, denoise as D
     if not D.WEIGHTS.exists():
         log("denoise: no trained weights in the package, leg skipped")
@@ -106,8 +106,9 @@ def denoise_leg(ctx, sc, ref_spp, save_dir=None):
         log(f"denoise: reference half {len(halves)}/2 ({ref_spp // 2} spp, {time.time() - t0:.0f}s)")
     ref = 0.5 * (halves[0] + halves[1])
     ref[..., 3] = 1.0
-    # relMSE(half A, half B) / 4 estimates relMSE(truth, ref) for the averaged reference
-    floor = 0.25 * D.relmse(halves[0], halves[1])
+    # disagreement of the two independent halves (heavy-tailed sun caustics make
+    # the reference itself noisy; reported, not used)
+    halves_relmse = D.relmse(halves[0], halves[1])
     ld = D.LearnedDenoiser(D.WEIGHTS)
     untrained = ohpt.Denoiser(seed=1)
     ctx.set_render_seed(1)
@@ -153,7 +154,7 @@ def denoise_leg(ctx, sc, ref_spp, save_dir=None):
     achieved = conv_flops / (ms_conv * 1e-3) / 1e12 if ms_conv > 0 else 0.0
     res = {"metric": "GPU time to equal relMSE, denoised 1 spp vs plain PT",
            "reference": f"{ref_spp} spp as two independent halves (render seeds 50000000, 60000000)",
-           "reference_noise_floor_relmse": round(floor, 6), "relmse_eps": D.RELMSE_EPS,
+           "reference_halves_relmse": round(halves_relmse, 6), "relmse_eps": D.RELMSE_EPS,
            "relmse_plain": {str(c): round(v["relmse"], 6) for c, v in plain.items()},
            "ldr_rmse_plain": {str(c): round(v["ldr_rmse"], 6) for c, v in plain.items()},
            "gpu_ms_plain": {str(c): v["gpu_ms"] for c, v in plain.items()},
@@ -170,7 +171,7 @@ def denoise_leg(ctx, sc, ref_spp, save_dir=None):
                         "kernel": "k_conv3x3 (BF16 MFMA implicit GEMM, 19 convolutions)",
                         "conv_gflop_per_inference": round(conv_flops / 1e9, 2), "conv_ms": round(ms_conv, 3)}}
     log(f"denoise: relMSE denoised {e_den:.5f} (untrained {e_untrained:.5f}, 1 spp {plain[1]['relmse']:.5f}, "
-        f"1024 spp {plain[1024]['relmse']:.5f}, reference floor ~{floor:.5f}); LDR RMSE denoised {l_den:.5f} "
+        f"1024 spp {plain[1024]['relmse']:.5f}, reference halves {halves_relmse:.5f}); LDR RMSE denoised {l_den:.5f} "
         f"vs 1024 spp {plain[1024]['ldr_rmse']:.5f}; GPU {t_den:.2f} ms vs {plain[1024]['gpu_ms']:.1f} ms")
     return res
 
