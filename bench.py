@@ -272,23 +272,37 @@ def main():
 
     # path-replay adjoint of the same frame: dL/dI = per-pixel weights, gradients
     # scattered into the 1024x512 ground albedo texture
+    # (dL/dI resident in HBM, like the scene)
+    import torch
     adj_steps = max(1, a.steps // 2)
-    w = np.random.default_rng(0).uniform(0.5, 1.5, (sc.height, sc.width, 3)).astype(np.float32)
-    ctx.zero_gradients()
-    ctx.adjoint(w, spp)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    adj_ms = 0.0
-    for i in range(adj_steps):
-        ctx.adjoint(w, spp)
-        adj_ms += ctx.stats().ms_total
-    adj_wall = time.perf_counter() - t0
+    w = np.zeros((sc.height, sc.width, 4), np.float32)
+    w[..., :3] = np.random.default_rng(0).uniform(0.5, 1.5, (sc.height, sc.width, 3))
+    w_dev = torch.from_numpy(w).to("cuda:0")
+    torch.cuda.synchronize()
+
+    def time_adjoint(deterministic):
+        ctx.zero_gradients()
+        ctx.adjoint(w_dev, spp, deterministic=deterministic)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        gpu = 0.0
+        for i in range(adj_steps):
+            ctx.adjoint(w_dev, spp, deterministic=deterministic)
+            gpu += ctx.stats().ms_total
+        ctx.synchronize()
+        return time.perf_counter() - t0, gpu
+
+    adj_wall, adj_ms = time_adjoint(False)
+    det_wall, det_ms = time_adjoint(True)
     adjoint = {"value": round(sc.width * sc.height * spp * adj_steps / adj_wall / 1e6, 3), "unit": "Mpaths/s",
                "ms_per_step": round(adj_wall * 1e3 / adj_steps, 4),
                "gpu_ms_per_step": round(adj_ms / adj_steps, 4),
                "cost_vs_forward": round((adj_wall / adj_steps) / (wall / a.steps), 3),
-               "parameters": "1024x512 RGB ground albedo texture", "mode": "fp32 atomics"}
-    log(f"adjoint: {adjoint['ms_per_step']} ms/step, {adjoint['cost_vs_forward']}x forward")
+               "parameters": "1024x512 RGB ground albedo texture", "mode": "fp32 atomics",
+               "deterministic_ms_per_step": round(det_wall * 1e3 / adj_steps, 4),
+               "deterministic_gpu_ms_per_step": round(det_ms / adj_steps, 4)}
+    log(f"adjoint: {adjoint['ms_per_step']} ms/step, {adjoint['cost_vs_forward']}x forward; deterministic "
+        f"{adjoint['deterministic_ms_per_step']} ms/step")
 
     denoise = None
     if not a.no_denoise:

@@ -1,0 +1,213 @@
+// inverse_fit driver over the C++ façade: fits a sphere's albedo texture to
+// multi-view target images with path-replay adjoint gradients and Adam — the
+// gradient source the reference's inverse lab gets from coordinate finite
+// differences (ohao/inverse/staged_fit.hpp:296-357). Adam is the reference's
+// AdamState::step (ohao/inverse/optimizer.hpp:26-40) with projection to [0, 1].
+//
+// Scene: UV sphere (2 x seg^2 triangles) on a plane, RGB albedo texture whose
+// ground truth is a seeded 16x16-tile checker with U[0.1, 0.9] tiles; the fit
+// starts from a uniform 0.5 texture. Views on a 30-degree ring, L2 loss on the
+// mean HDR radiance.
+//
+//   inverse_fit [--views=8] [--size=512] [--spp=16] [--tex=1024] [--seg=316] [--iters=20]
+//               [--lr=0.02] [--target-spp=256] [--selftest]
+// Each step renders fresh samples (stochastic gradients); the learning rate
+// decays with a cosine schedule to 10 %.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "ohao/renderer.hpp"
+
+using namespace ohao;
+
+namespace {
+
+int arg_int(const std::string& a, const char* key, int v) {
+    const std::string k = std::string("--") + key + "=";
+    return a.rfind(k, 0) == 0 ? std::atoi(a.c_str() + k.size()) : v;
+}
+
+struct Adam {
+    std::vector<double> m, v;
+    int t = 0;
+    void step(std::vector<float>& x, const std::vector<float>& g, double lr) {
+        if (m.size() != x.size()) { m.assign(x.size(), 0.0); v.assign(x.size(), 0.0); }
+        ++t;
+        const double b1t = 1.0 - std::pow(0.9, t), b2t = 1.0 - std::pow(0.999, t);
+        for (size_t i = 0; i < x.size(); ++i) {
+            m[i] = 0.9 * m[i] + 0.1 * g[i];
+            v[i] = 0.999 * v[i] + 0.001 * (double)g[i] * g[i];
+            const double step = lr * (m[i] / b1t) / (std::sqrt(v[i] / b2t) + 1e-8);
+            x[i] = (float)std::clamp((double)x[i] - step, 0.0, 1.0);
+        }
+    }
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    int views = 8, size = 512, spp = 16, tex = 1024, seg = 316, iters = 20, target_spp = 256;
+    double lr = 0.02;
+    bool selftest = false;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        views = arg_int(a, "views", views);
+        size = arg_int(a, "size", size);
+        spp = arg_int(a, "spp", spp);
+        tex = arg_int(a, "tex", tex);
+        seg = arg_int(a, "seg", seg);
+        iters = arg_int(a, "iters", iters);
+        target_spp = arg_int(a, "target-spp", target_spp);
+        if (a.rfind("--lr=", 0) == 0) lr = std::atof(a.c_str() + 5);
+        if (a == "--selftest") selftest = true;
+    }
+    if (selftest) {   // small and quick: a loss that must fall
+        views = 2; size = 128; spp = 8; tex = 256; seg = 64; iters = 6; target_spp = 64;
+    }
+    std::printf("ohao inverse_fit: %d views %dx%d, %d spp, %dx%d texture, %d iterations\n", views, size, size, spp,
+                tex, tex, iters);
+
+    VulkanRenderer r((uint32_t)size, (uint32_t)size);
+    if (!r.initialize()) {
+        std::fprintf(stderr, "FATAL: %s\n", r.lastError().c_str());
+        return 1;
+    }
+    Scene scene("InverseFit");
+    // ground-truth checker and the initial guess
+    std::mt19937 rng(3);
+    std::uniform_real_distribution<float> U(0.1f, 0.9f);
+    std::vector<float> tiles(16 * 16 * 3);
+    for (float& t : tiles) t = U(rng);
+    Texture gt;
+    gt.width = gt.height = (uint32_t)tex;
+    gt.channels = 3;
+    gt.texels.resize((size_t)tex * tex * 3);
+    for (int y = 0; y < tex; ++y)
+        for (int x = 0; x < tex; ++x)
+            for (int c = 0; c < 3; ++c)
+                gt.texels[((size_t)y * tex + x) * 3 + c] = tiles[((y * 16 / tex) * 16 + x * 16 / tex) * 3 + c];
+    const std::vector<float> truth = gt.texels;
+    const int32_t tid = scene.addTexture(gt);
+
+    auto ball = scene.createActor("Lantern");
+    ball->addComponent<MeshComponent>()->setModel(makeSphere(1.0f, seg));
+    ball->getTransform()->setPosition({0.0f, 1.0f, 0.0f});
+    Material& bm = ball->addComponent<MaterialComponent>()->getMaterial();
+    bm.baseColor = vec3(1.0f);
+    bm.roughness = 0.6f;
+    bm.albedoTexture = tid;
+    auto floor = scene.createActor("Floor");
+    floor->addComponent<MeshComponent>()->setModel(
+        makeQuad({-6, 0, -6}, {-6, 0, 6}, {6, 0, 6}, {6, 0, -6}, {0, 1, 0}));
+    floor->addComponent<MaterialComponent>()->getMaterial().baseColor = vec3(0.6f);
+    auto light = scene.createActor("Panel");
+    auto lc = light->addComponent<LightComponent>();
+    lc->setLightType(LightType::AreaRect);
+    lc->setColor({1.0f, 0.97f, 0.9f});
+    lc->setIntensity(6.0f);
+    lc->setAreaEdges({2, 0, 0}, {0, 0, 2});
+    light->getTransform()->setPosition({-1.0f, 4.0f, -1.0f});
+    r.setScene(&scene);
+    std::vector<float> env((size_t)256 * 128 * 4, 1.0f);
+    for (int y = 0; y < 128; ++y)
+        for (int x = 0; x < 256; ++x)
+            for (int c = 0; c < 3; ++c) env[((size_t)y * 256 + x) * 4 + c] = 0.05f + 0.55f * (1.0f - y / 127.0f);
+    r.setEnvironmentPixels(env, 256, 128);
+    RTRenderSettings st;
+    st.maxBounces = 4;
+    st.rrStart = 1000;   // no Russian roulette: the estimator is smooth in the albedo
+    st.sobol = false;
+    r.setRTRenderSettings(st);
+    r.setFrameLatency(0);
+
+    struct View { vec3 pos; float pitch, yaw; };
+    std::vector<View> cams;
+    for (int k = 0; k < views; ++k) {
+        const float a = 6.2831853f * k / views, el = radians(30.0f);
+        const vec3 p(4.0f * std::cos(el) * std::cos(a), 1.0f + 4.0f * std::sin(el), 4.0f * std::cos(el) * std::sin(a));
+        const vec3 d = normalize(vec3(0.0f, 1.0f, 0.0f) - p);
+        cams.push_back({p, degrees(std::asin(d.y)), degrees(std::atan2(d.z, d.x))});
+    }
+    const size_t npix = (size_t)size * size;
+    auto render = [&](int v, uint32_t seed, int n, std::vector<float>& img) {
+        r.getCamera().setPosition(cams[(size_t)v].pos);
+        r.getCamera().setRotation(cams[(size_t)v].pitch, cams[(size_t)v].yaw);
+        r.getCamera().setFov(40.0f);
+        r.setRenderSeed(seed);
+        r.resetAccumulation();
+        std::vector<float> alb, nrm;
+        uint32_t w, h;
+        return r.renderSamples((uint32_t)n) && r.readbackHDRBuffers(img, alb, nrm, w, h);
+    };
+    std::vector<std::vector<float>> targets((size_t)views);
+    for (int v = 0; v < views; ++v)
+        if (!render(v, 5000000u + 10000u * (uint32_t)v, target_spp, targets[(size_t)v])) {
+            std::fprintf(stderr, "FATAL: target render: %s\n", r.lastError().c_str());
+            return 1;
+        }
+    Texture& param = scene.getTexture(tid);
+    std::fill(param.texels.begin(), param.texels.end(), 0.5f);
+    if (!r.updateTexture(tid)) {
+        std::fprintf(stderr, "FATAL: %s\n", r.lastError().c_str());
+        return 1;
+    }
+    Adam adam;
+    std::vector<float> img, grad, dl(npix * 4);
+    std::vector<char> visible;
+    double first_loss = -1.0, loss = 0.0;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int it = 0; it < iters; ++it) {
+        loss = 0.0;
+        r.zeroGradients();
+        for (int v = 0; v < views; ++v) {
+            const uint32_t seed = 1000u + (uint32_t)v + 100003u * (uint32_t)(it + 1);
+            if (!render(v, seed, spp, img)) return 1;
+            const auto& tg = targets[(size_t)v];
+            for (size_t i = 0; i < npix; ++i) {
+                for (int c = 0; c < 3; ++c) {
+                    const double d = (double)img[i * 4 + c] - tg[i * 4 + c];
+                    loss += d * d / (3.0 * npix);
+                    dl[i * 4 + c] = (float)(2.0 * d / (3.0 * npix));
+                }
+                dl[i * 4 + 3] = 0.0f;
+            }
+            r.setRenderSeed(seed);
+            if (!r.adjoint(dl, (uint32_t)spp)) {
+                std::fprintf(stderr, "FATAL: adjoint: %s\n", r.lastError().c_str());
+                return 1;
+            }
+        }
+        if (!r.textureGradient(tid, grad)) return 1;
+        if (visible.empty()) {
+            visible.resize(grad.size());
+            for (size_t i = 0; i < grad.size(); ++i) visible[i] = grad[i] != 0.0f;
+        }
+        const double lr_t = lr * (0.1 + 0.45 * (1.0 + std::cos(3.141592653589793 * it / std::max(iters - 1, 1))));
+        adam.step(param.texels, grad, lr_t);
+        if (!r.updateTexture(tid)) return 1;
+        double se = 0.0;
+        size_t n = 0;
+        for (size_t i = 0; i < truth.size(); ++i)
+            if (visible[i]) {
+                const double d = param.texels[i] - truth[i];
+                se += d * d;
+                ++n;
+            }
+        if (first_loss < 0) first_loss = loss;
+        std::printf("iter %d/%d: loss %.6f visible-texel RMSE %.4f\n", it + 1, iters, loss,
+                    std::sqrt(se / (double)std::max<size_t>(n, 1)));
+        std::fflush(stdout);
+    }
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("Done: %.1f s, loss %.6f -> %.6f\n", s, first_loss, loss);
+    if (selftest && !(loss < 0.7 * first_loss)) {
+        std::fprintf(stderr, "selftest: loss did not fall enough\n");
+        return 2;
+    }
+    return 0;
+}

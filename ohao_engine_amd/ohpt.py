@@ -125,6 +125,14 @@ def lib() -> C.CDLL:
         if not LIB_PATH.exists():
             raise OhptError(f"{LIB_PATH} is missing: build it with `python -m ohao_engine_amd.build` "
                             "(the path tracer has no CPU fallback)")
+        # PyTorch-ROCm bundles its own HIP runtime under the same soname. Loaded
+        # first, it serves this library too (one runtime per process); loaded
+        # after this library, torch would start a second runtime that finds no
+        # device. So torch, when installed, is imported before the library.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)

@@ -57,3 +57,25 @@ def test_struct_layouts_match_header():
     assert C.sizeof(ohpt.Settings) == 32
     assert C.sizeof(ohpt.Stats) == 88
     assert scenes.MATERIAL_DTYPE.itemsize == 64
+
+
+def test_denoiser_abi_without_gpu(ohpt_lib):
+    from ohao_engine_amd import ohpt
+    L = ohpt_lib
+    assert C.sizeof(ohpt.DenoiserStats) == 32
+    assert C.sizeof(ohpt.DenoiseInputs) == 4 * C.sizeof(C.c_void_p)
+    assert L.ohpt_denoiser_create(None, 0, 0) == -1
+    assert L.ohpt_denoiser_destroy(None) == 0
+    assert L.ohpt_denoiser_infer(None, None, None, 1, 4, 4, 0) == -1
+    assert L.ohpt_denoiser_adam_step(None, 1e-3) == -1
+    assert L.ohpt_denoise(None, None, None, 0) == -1
+    n = C.c_uint64()
+    assert L.ohpt_denoiser_param_count(None, C.byref(n)) == -1
+
+
+def test_facade_library_and_drivers_are_built():
+    pkg = ROOT / "ohao_engine_amd"
+    lib = C.CDLL(str(pkg / "libohao.so"))
+    assert lib is not None
+    for exe in ("cornell_box", "turntable", "inverse_fit"):
+        assert (pkg / "bin" / exe).exists()

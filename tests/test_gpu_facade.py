@@ -67,3 +67,8 @@ def test_learned_denoise_mode_through_facade(tmp_path):
     out = run([BIN / "cornell_box", tmp_path / "dn.png", 4, "--size=64x64", "--denoise=learned"], tmp_path)
     assert "Denoise mode: Learned" in out
     assert (tmp_path / "dn.png").stat().st_size > 64 * 64 * 4
+
+
+def test_inverse_fit_driver_selftest(tmp_path):
+    out = run([BIN / "inverse_fit", "--selftest"], tmp_path, timeout=300)
+    assert "Done:" in out
