@@ -122,16 +122,17 @@ typedef struct ohpt_camera {
 /* GPU timing of the last ohpt_render / ohpt_adjoint call (HIP events). */
 typedef struct ohpt_stats {
     double ms_total;               /* generate -> accumulate                    */
-    double ms_extend;              /* closest-hit traversal kernels             */
+    double ms_extend;              /* traversal kernels: closest-hit rays and, in ohpt_render,
+                                      the shadow rays traced in the same launches */
     double ms_shade;
-    double ms_connect;             /* shadow-ray kernels                        */
+    double ms_connect;             /* separate shadow-ray kernels (ohpt_adjoint) */
     double ms_build;               /* last LBVH build                           */
     uint64_t rays_extend;          /* closest-hit rays traced                   */
     uint64_t rays_shadow;          /* shadow rays traced                        */
     uint64_t paths;
-    uint64_t node_visits;          /* extend-kernel BVH node visits (OHPT_RENDER_PROFILE only) */
-    uint64_t tri_tests;            /* extend-kernel ray/triangle tests (OHPT_RENDER_PROFILE only) */
-    uint32_t extend_launches;      /* extend-kernel launches in the last call   */
+    uint64_t node_visits;          /* traversal BVH node visits, all rays (OHPT_RENDER_PROFILE only) */
+    uint64_t tri_tests;            /* ray/triangle tests, all rays (OHPT_RENDER_PROFILE only) */
+    uint32_t extend_launches;      /* traversal launches in the last call       */
     uint32_t reserved;
 } ohpt_stats;
 
@@ -206,7 +207,9 @@ int ohpt_set_render_seed(ohpt_context* ctx, uint32_t seed);
  * (render_dispatch.cpp:139-141) results are visible right after the call. */
 enum {
     OHPT_RENDER_ASYNC = 1u << 0,    /* return without waiting for the GPU          */
-    OHPT_RENDER_PROFILE = 1u << 1   /* time extend/shade/connect with HIP events    */
+    OHPT_RENDER_PROFILE = 1u << 1   /* time every kernel with HIP events and count node
+                                       visits; the frame then runs as one wavefront on
+                                       the context stream (no concurrent lanes)    */
 };
 int ohpt_render(ohpt_context* ctx, uint32_t spp, uint32_t render_flags);
 int ohpt_synchronize(ohpt_context* ctx);

@@ -100,6 +100,11 @@ struct ohpt_context {
 
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::vector<hipEvent_t> ev_pool;
+    // A frame's paths split into kLanes independent wavefronts on their own
+    // streams, so the tail of one lane's traversal overlaps the other's work.
+    static constexpr int kLanes = 2;
+    hipStream_t lane_stream[kLanes] = {};
+    hipEvent_t ev_fork = nullptr, ev_lane[kLanes] = {};
     ohpt_stats stats{};
 };
 
@@ -166,11 +171,30 @@ int ensure_paths(ohpt_context* c, uint32_t n) {
     HIPCHK(c->sh_o.ensure(2 * N * 16));
     HIPCHK(c->sh_d.ensure(2 * N * 16));
     HIPCHK(c->sh_c.ensure(2 * N * 16));
-    HIPCHK(c->counters.ensure(kCounterWords * 4));
-    HIPCHK(c->ovf.ensure(trace_overflow_bytes()));
+    HIPCHK(c->counters.ensure((size_t)kCounterWords * 4 * ohpt_context::kLanes));
+    HIPCHK(c->ovf.ensure(trace_overflow_bytes() * ohpt_context::kLanes));
     c->ovf_rays = 2 * n;
     c->capacity = n;
     return OHPT_OK;
+}
+
+// Lane `lane` of a batch: paths [pid_base, pid_base + n) share the per-path
+// arrays; its queues, hits and shadow rays use the matching ranges of the
+// batch-sized buffers, its counters and stack overflow slots are its own.
+PathState lane_view(const PathState& ps, uint32_t pid_base, int lane) {
+    PathState p = ps;
+    p.q[0] = ps.q[0] + pid_base;
+    p.q[1] = ps.q[1] + pid_base;
+    p.hits = ps.hits + pid_base;
+    p.sh_o = ps.sh_o + 2 * (size_t)pid_base;
+    p.sh_d = ps.sh_d + 2 * (size_t)pid_base;
+    p.sh_c = ps.sh_c + 2 * (size_t)pid_base;
+    p.counters = ps.counters + (size_t)lane * kCounterWords;
+    return p;
+}
+
+uint32_t* lane_ovf(ohpt_context* c, int lane) {
+    return c->ovf.as<uint32_t>() + (size_t)lane * (trace_overflow_bytes() / 4);
 }
 
 PathState path_state(ohpt_context* c) {
@@ -307,9 +331,14 @@ int ohpt_create(ohpt_context** out, int device, uint32_t width, uint32_t height)
     c->device = device;
     c->W = width;
     c->H = height;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
-        delete c;
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&c->ev_start) == hipSuccess && hipEventCreate(&c->ev_stop) == hipSuccess &&
+              hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int l = 0; ok && l < ohpt_context::kLanes; ++l)
+        ok = hipStreamCreateWithFlags(&c->lane_stream[l], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_lane[l], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        ohpt_destroy(c);
         return fail(OHPT_ERR_HIP, "ohpt_create: stream/event creation failed");
     }
     // reference offline defaults: maxBounces 4, Sobol sampler, RR after bounce 1 (rt_settings.hpp:50-61)
@@ -333,9 +362,16 @@ int ohpt_destroy(ohpt_context* c) {
     if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_destroy: null context");
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int l = 0; l < ohpt_context::kLanes; ++l)
+        if (c->lane_stream[l]) (void)hipStreamSynchronize(c->lane_stream[l]);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    for (int l = 0; l < ohpt_context::kLanes; ++l) {
+        if (c->ev_lane[l]) (void)hipEventDestroy(c->ev_lane[l]);
+        if (c->lane_stream[l]) (void)hipStreamDestroy(c->lane_stream[l]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OHPT_OK;
@@ -736,37 +772,65 @@ int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
         f.flags = c->st.flags;
         f.firefly_clamp = c->st.firefly_clamp;
         f.ray_eps = c->st.ray_eps;
-        launch_fill_u32(ps.counters, 0u, kCounterWords, c->stream);
-        launch_generate(f, ps, c->stream);
-        for (uint32_t b = 0; b <= f.max_bounces; ++b) {
+        // Lanes: two independent wavefronts on their own streams once the batch
+        // is large enough; one lane on the context stream when profiling, so
+        // the per-kernel events time each kernel alone.
+        const int nl = (!profile && f.n_paths >= (1u << 18)) ? ohpt_context::kLanes : 1;
+        const uint32_t split = nl == 1 ? f.n_paths : ((f.n_paths / 2 + 63u) & ~63u);
+        if (nl > 1) HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+        for (int l = 0; l < nl; ++l) {
+            const hipStream_t s = nl == 1 ? c->stream : c->lane_stream[l];
+            if (nl > 1) HIPCHK(hipStreamWaitEvent(s, c->ev_fork, 0));
+            FrameParams fl = f;
+            fl.pid_base = l == 0 ? 0u : split;
+            fl.n_paths = l == 0 ? split : f.n_paths - split;
+            const PathState pl = lane_view(ps, fl.pid_base, l);
+            uint32_t* ovf = lane_ovf(c, l);
+            launch_fill_u32(pl.counters, 0u, kCounterWords, s);
+            launch_generate(fl, pl, s);
+            // Extension rays of bounce b+1 and shadow rays of bounce b share one
+            // traversal launch: both come from shade(b) and are independent.
             Seg sg{};
             if ((r = mark(&sg.a))) return r;
-            launch_extend(S, ps.q[b & 1u], ps.counters + b, f.n_paths, ps.o, ps.d, ps.hits,
-                          ps.counters + kExtendFetchBase + b, c->ovf.as<uint32_t>(), tstats, c->stream);
+            launch_extend(S, pl.q[0], pl.counters, fl.n_paths, pl.o, pl.d, pl.hits, pl.counters + kExtendFetchBase,
+                          ovf, tstats, s);
             ++n_extend;
             if ((r = mark(&sg.b))) return r;
             if (profile) { sg.kind = 0; segs.push_back(sg); }
-            if ((r = mark(&sg.a))) return r;
-            launch_shade(S, f, ps, b, c->stream);
-            if ((r = mark(&sg.b))) return r;
-            if (profile) { sg.kind = 1; segs.push_back(sg); }
-            if (b < f.max_bounces) {
+            for (uint32_t b = 0; b <= f.max_bounces; ++b) {
                 if ((r = mark(&sg.a))) return r;
-                launch_connect(S, ps.counters + kShadowCounterBase + b, 2u * f.n_paths, ps.sh_o, ps.sh_d, ps.sh_c,
-                               ps.ln0, ps.ln1, ps.counters + kConnectFetchBase + b, c->ovf.as<uint32_t>(),
-                               nullptr, b, c->stream);
+                launch_shade(S, fl, pl, b, s);
                 if ((r = mark(&sg.b))) return r;
-                if (profile) { sg.kind = 2; segs.push_back(sg); }
+                if (profile) { sg.kind = 1; segs.push_back(sg); }
+                if (b < f.max_bounces) {
+                    if ((r = mark(&sg.a))) return r;
+                    launch_extend_connect(S, pl.q[(b + 1) & 1u], pl.counters + b + 1, fl.n_paths, pl.o, pl.d,
+                                          pl.hits, pl.counters + kShadowCounterBase + b, 2u * fl.n_paths, pl.sh_o,
+                                          pl.sh_d, pl.sh_c, pl.ln0, pl.ln1, pl.counters + kExtendFetchBase + b + 1,
+                                          ovf, b, tstats, s);
+                    ++n_extend;
+                    if ((r = mark(&sg.b))) return r;
+                    if (profile) { sg.kind = 0; segs.push_back(sg); }
+                }
+            }
+            if (nl > 1) {
+                HIPCHK(hipEventRecord(c->ev_lane[l], s));
+                HIPCHK(hipStreamWaitEvent(c->stream, c->ev_lane[l], 0));
             }
         }
         launch_accumulate(f, ps, c->d_accum.as<float4>(), c->d_moments.as<float4>(), c->d_aov_albedo.as<float4>(),
                           c->d_aov_normal.as<float4>(), c->stream);
         HIPCHK(hipGetLastError());
         if (!(render_flags & OHPT_RENDER_ASYNC)) {
-            HIPCHK(hipMemcpyAsync(host_counters.data(), ps.counters, kCounterWords * 4, hipMemcpyDeviceToHost, c->stream));
+            host_counters.resize((size_t)kCounterWords * nl);
+            HIPCHK(hipMemcpyAsync(host_counters.data(), ps.counters, (size_t)kCounterWords * 4 * nl,
+                                  hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
-            for (uint32_t b = 0; b <= f.max_bounces; ++b) rays_ext += host_counters[b];
-            for (uint32_t b = 0; b < f.max_bounces; ++b) rays_sh += host_counters[kShadowCounterBase + b];
+            for (int l = 0; l < nl; ++l) {
+                const uint32_t* hc = host_counters.data() + (size_t)l * kCounterWords;
+                for (uint32_t b = 0; b <= f.max_bounces; ++b) rays_ext += hc[b];
+                for (uint32_t b = 0; b < f.max_bounces; ++b) rays_sh += hc[kShadowCounterBase + b];
+            }
         }
         c->sample_index += batch;
         remaining -= batch;

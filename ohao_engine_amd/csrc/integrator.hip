@@ -43,9 +43,10 @@ __global__ __launch_bounds__(256) void k_generate(FrameParams f, float4* __restr
                                                   float4* __restrict__ T, float4* __restrict__ L,
                                                   float4* __restrict__ ln0, float4* __restrict__ ln1,
                                                   uint32_t* __restrict__ q, uint32_t* __restrict__ counters) {
-    const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
-    if (pid == 0) counters[0] = f.n_paths;
-    if (pid >= f.n_paths) return;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) counters[0] = f.n_paths;
+    if (i >= f.n_paths) return;
+    const uint32_t pid = f.pid_base + i;
     const uint32_t npix = f.width * f.height;
     const uint32_t pixel = pid % npix, k = pid / npix;
     const uint32_t px = pixel % f.width, py = pixel / f.width;
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(256) void k_generate(FrameParams f, float4* __restr
     L[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     ln0[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     ln1[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    q[pid] = pid;
+    q[i] = pid;
 }
 
 struct SurfaceHit {

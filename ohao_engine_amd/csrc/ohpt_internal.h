@@ -35,10 +35,20 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t
                             hipStream_t s);
 
 // ---- traversal (traverse.hip) ---------------------------------------------
-constexpr uint32_t kTraceBlock = 128;
-constexpr uint32_t kStackLds = 32;       // per-thread LDS stack entries
-constexpr uint32_t kStackOvf = 32;       // per-thread global overflow entries
-constexpr uint32_t kMaxTreeDepth = 60;   // register(4) + LDS + overflow comfortably above this
+// Launch shape of the traversal kernels (compile-time; -D overrides for tuning sweeps).
+#ifndef OHPT_TRACE_BLOCK
+#define OHPT_TRACE_BLOCK 128
+#endif
+#ifndef OHPT_STACK_LDS
+#define OHPT_STACK_LDS 32
+#endif
+constexpr uint32_t kTraceBlock = OHPT_TRACE_BLOCK;
+constexpr uint32_t kStackLds = OHPT_STACK_LDS;   // per-thread LDS stack entries
+constexpr uint32_t kStackOvf = 48;               // per-thread global overflow entries
+constexpr uint32_t kMaxTreeDepth = 60;   // commit rejects deeper trees
+// at most one push per tree level: registers (4) + LDS + overflow must cover the deepest tree
+static_assert(4 + kStackLds + kStackOvf >= kMaxTreeDepth, "traversal stack shallower than the deepest tree");
+static_assert(kTraceBlock % 64 == 0 && kTraceBlock <= 1024, "trace block must be whole waves");
 
 // Persistent-thread grid of the traversal kernels and the size of their
 // global stack-overflow buffer (one kStackOvf slot per resident thread).
@@ -56,6 +66,13 @@ void launch_extend(const DevScene& S, const uint32_t* queue, const uint32_t* cou
 void launch_connect(const DevScene& S, const uint32_t* count, uint32_t max_count, const float4* sh_o,
                     const float4* sh_d, const float4* sh_c, float4* ln0, float4* ln1, uint32_t* fetch,
                     uint32_t* ovf, uint32_t* vis, uint32_t bounce, hipStream_t s);
+// One launch for the extension rays of bounce b+1 (closest hit, into `hits`)
+// and the shadow rays of bounce b (any hit, as launch_connect without `vis`).
+void launch_extend_connect(const DevScene& S, const uint32_t* queue, const uint32_t* count_ext, uint32_t max_ext,
+                           const float4* ray_o, const float4* ray_d, float4* hits, const uint32_t* count_sh,
+                           uint32_t max_sh, const float4* sh_o, const float4* sh_d, const float4* sh_c, float4* ln0,
+                           float4* ln1, uint32_t* fetch, uint32_t* ovf, uint32_t bounce,
+                           unsigned long long* stats, hipStream_t s);
 // Caller rays (ox,oy,oz,tmin,dx,dy,dz,tmax) -> (t, prim|miss, u, v) / occluded flags.
 void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* fetch,
                        uint32_t* ovf, hipStream_t s);
@@ -66,8 +83,9 @@ void launch_occluded_rays(const DevScene& S, const float4* rays, uint32_t* out, 
 struct FrameParams {
     ohpt_camera cam;
     uint32_t width, height;
-    uint32_t n_paths;          // width*height*spp_batch
+    uint32_t n_paths;          // paths of this launch (a lane: a contiguous range of the batch)
     uint32_t sample_base;      // sample index of the first sample of this batch
+    uint32_t pid_base;         // first path id of the launch; pid = k*W*H + pixel over the whole batch
     uint32_t max_bounces, rr_start, sampler, flags;
     float firefly_clamp, ray_eps;
 };

@@ -26,7 +26,10 @@ namespace {
 
 constexpr float kGamma3Scale = 1.0f + 2.0f * (3.0f * 5.9604645e-08f) / (1.0f - 3.0f * 5.9604645e-08f);
 // lanes keep stepping without refilling while at least this many are busy
-constexpr int kRefillThreshold = 48;
+#ifndef OHPT_REFILL_THRESHOLD
+#define OHPT_REFILL_THRESHOLD 16
+#endif
+constexpr int kRefillThreshold = OHPT_REFILL_THRESHOLD;
 
 struct RayPre {
     float ox, oy, oz;
@@ -201,7 +204,25 @@ struct RaysIO {
     }
 };
 
-template <bool kAnyHit, class IO>
+// kMode: 0 closest hit, 1 any hit, 2 per ray (IO::any_hit(j)).
+// Extension rays of one bounce and shadow rays of the previous one, traced by
+// one launch: indices [0, n_ext) are closest-hit rays, the rest any-hit.
+struct FusedIO {
+    ExtendIO ext;
+    ConnectIO con;
+    uint32_t n_ext;
+    __device__ __forceinline__ bool any_hit(uint32_t j) const { return j >= n_ext; }
+    __device__ __forceinline__ void load(uint32_t j, float3& o, float3& d, float& tmin, float& tmax) const {
+        if (j < n_ext) ext.load(j, o, d, tmin, tmax);
+        else con.load(j - n_ext, o, d, tmin, tmax);
+    }
+    __device__ __forceinline__ void store(const DevScene& S, uint32_t j, float4 h) const {
+        if (j < n_ext) ext.store(S, j, h);
+        else con.store(S, j - n_ext, h);
+    }
+};
+
+template <int kMode, class IO>
 __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io, uint32_t count,
                                                  uint32_t* __restrict__ fetch, uint32_t* lds_col, uint32_t* ovf,
                                                  unsigned long long* stats) {
@@ -213,8 +234,31 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
     uint32_t cur = 0, j = 0;
     float tmin = 0.0f, tclose = 0.0f;
     float4 hit = make_float4(0, 0, 0, 0);
-    bool active = false, drained = false;
+    bool active = false, anyhit = kMode == 1;
     uint32_t nv = 0, nt = 0;
+    // The first ray of every thread is its global index (no atomics: launches
+    // with fewer rays than resident threads never touch the counter); the
+    // shared counter hands out the rays beyond the grid.
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    bool drained = nthreads >= count;
+    auto start = [&](uint32_t idx) {
+        float3 o, d;
+        float tmax;
+        j = idx;
+        if constexpr (kMode == 2) anyhit = io.any_hit(j);
+        io.load(j, o, d, tmin, tmax);
+        r = ray_setup(o, d);
+        tclose = tmax;
+        hit = make_float4(tmax, __uint_as_float(kMiss), 0.0f, 0.0f);
+        cur = 0;
+        st.reset();
+        active = S.n_tris > 0;
+        if (!active) io.store(S, j, hit);
+    };
+    {
+        const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+        if (g < count) start(g);
+    }
     for (;;) {
         // ---- refill idle lanes with new rays (one atomic per wave)
         if (!drained) {
@@ -224,23 +268,12 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
                 const uint32_t nw = (uint32_t)__popcll(want);
                 uint32_t base = 0;
                 if ((int)lane == leader) base = atomicAdd(fetch, nw);
-                base = __shfl(base, leader);
+                base = __shfl(base, leader) + nthreads;
                 if (base + nw >= count) drained = true;
                 if (!active) {
-                    j = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
-                    if (j < count) {
-                        float3 o, d;
-                        float tmax;
-                        io.load(j, o, d, tmin, tmax);
-                        r = ray_setup(o, d);
-                        tclose = tmax;
-                        hit = make_float4(tmax, __uint_as_float(kMiss), 0.0f, 0.0f);
-                        cur = 0;
-                        st.reset();
-                        active = S.n_tris > 0;
-                        if (!active) io.store(S, j, hit);
-                    }
+                    const uint32_t idx = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(want >> 32),
+                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)want, 0u));
+                    if (idx < count) start(idx);
                 }
             }
         }
@@ -275,7 +308,7 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
                         if (tri_hit(r, ta, tb, tc, tmin, tclose, tt, uu, vv)) {
                             tclose = tt;
                             hit = make_float4(tt, __uint_as_float(s), uu, vv);
-                            if (kAnyHit) done = true;
+                            if (anyhit) done = true;
                         }
                     }
                 }
@@ -316,14 +349,27 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DevScene S, ExtendIO io,
                                                         unsigned long long* __restrict__ stats) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
-    persistent_trace<false>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, stats);
+    persistent_trace<0>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, stats);
 }
 
 __global__ __launch_bounds__(kTraceBlock) void k_connect(DevScene S, ConnectIO io, const uint32_t* __restrict__ count,
                                                          uint32_t* __restrict__ fetch, uint32_t* __restrict__ ovf) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
-    persistent_trace<true>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
+    persistent_trace<1>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_extend_connect(DevScene S, FusedIO io,
+                                                                const uint32_t* __restrict__ count_ext,
+                                                                const uint32_t* __restrict__ count_sh,
+                                                                uint32_t* __restrict__ fetch,
+                                                                uint32_t* __restrict__ ovf,
+                                                                unsigned long long* __restrict__ stats) {
+    __shared__ uint32_t lds[kStackLds * kTraceBlock];
+    const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
+    io.n_ext = *count_ext;
+    persistent_trace<2>(S, io, io.n_ext + *count_sh, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf,
+                        stats);
 }
 
 template <bool kAnyHit>
@@ -331,7 +377,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_rays(DevScene S, RaysIO io, uin
                                                       uint32_t* __restrict__ ovf) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
-    persistent_trace<kAnyHit>(S, io, n, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
+    persistent_trace<kAnyHit ? 1 : 0>(S, io, n, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
 }
 
 __global__ void k_zero(uint32_t* p) { *p = 0u; }
@@ -371,6 +417,17 @@ void launch_connect(const DevScene& S, const uint32_t* count, uint32_t max_count
     ConnectIO io{sh_o, sh_d, sh_c, ln0, ln1, vis, bounce};
     const uint32_t grid = std::min(trace_grid_blocks(), (max_count + kTraceBlock - 1) / kTraceBlock);
     k_connect<<<grid, kTraceBlock, 0, s>>>(S, io, count, fetch, ovf);
+}
+
+void launch_extend_connect(const DevScene& S, const uint32_t* queue, const uint32_t* count_ext, uint32_t max_ext,
+                           const float4* ray_o, const float4* ray_d, float4* hits, const uint32_t* count_sh,
+                           uint32_t max_sh, const float4* sh_o, const float4* sh_d, const float4* sh_c, float4* ln0,
+                           float4* ln1, uint32_t* fetch, uint32_t* ovf, uint32_t bounce,
+                           unsigned long long* stats, hipStream_t s) {
+    if (max_ext + max_sh == 0) return;
+    FusedIO io{ExtendIO{queue, ray_o, ray_d, hits}, ConnectIO{sh_o, sh_d, sh_c, ln0, ln1, nullptr, bounce}, 0u};
+    const uint32_t grid = std::min(trace_grid_blocks(), (max_ext + max_sh + kTraceBlock - 1) / kTraceBlock);
+    k_extend_connect<<<grid, kTraceBlock, 0, s>>>(S, io, count_ext, count_sh, fetch, ovf, stats);
 }
 
 void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* fetch,

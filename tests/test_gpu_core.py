@@ -226,6 +226,30 @@ def test_render_is_deterministic():
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def test_lanes_and_fused_traversal_match_single_wavefront():
+    """512x512 at 1 spp (262,144 paths) runs as two concurrent lanes with the
+    shadow rays traced inside the next bounce's traversal launch; the profiled
+    mode runs one wavefront on one stream. Both images are identical bit for
+    bit, and so are the moments."""
+    sc = small_procedural()
+    sc.width = sc.height = 512
+    ctx = ohpt.Context(512, 512)
+    ctx.load_scene(sc)
+    out = []
+    for flags in (0, ohpt.RENDER_PROFILE):
+        ctx.set_render_seed(5)
+        ctx.reset()
+        ctx.render(2, flags)
+        s, q = ctx.read_moments()
+        out.append((s, q))
+        st = ctx.stats()
+        log(f"flags {flags}: {st.ms_total:.2f} ms, {st.rays_extend} extend + {st.rays_shadow} shadow rays, "
+            f"{st.extend_launches} traversal launches")
+    ctx.close()
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
+
+
 def test_sobol_sampler_runs_and_matches_oracle(oracle):
     sc = S.cornell_box(width=64, height=64)
     sc.settings.sampler = S.SAMPLER_SOBOL_OWEN
