@@ -31,7 +31,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mpaths/s fwd & adjoint @1080p; GPU time to equal relMSE, denoised vs plain PT"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = ROOT / "profiles" / "extend_traffic_procedural_1080p.json"
+TRAFFIC_FILE = ROOT / "profiles" / "traversal_traffic_procedural_1080p.json"
 
 
 def log(msg):

@@ -807,7 +807,7 @@ int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
                     launch_extend_connect(S, pl.q[(b + 1) & 1u], pl.counters + b + 1, fl.n_paths, pl.o, pl.d,
                                           pl.hits, pl.counters + kShadowCounterBase + b, 2u * fl.n_paths, pl.sh_o,
                                           pl.sh_d, pl.sh_c, pl.ln0, pl.ln1, pl.counters + kExtendFetchBase + b + 1,
-                                          ovf, b, tstats, s);
+                                          ovf, nullptr, b, tstats, s);
                     ++n_extend;
                     if ((r = mark(&sg.b))) return r;
                     if (profile) { sg.kind = 0; segs.push_back(sg); }
@@ -968,30 +968,47 @@ int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_
         f.flags = c->st.flags & ~(uint32_t)OHPT_FLAG_AOVS;
         f.firefly_clamp = c->st.firefly_clamp;
         f.ray_eps = c->st.ray_eps;
-        // pass 1: forward, recording shadow visibility
-        launch_fill_u32(ps.counters, 0u, kCounterWords, c->stream);
-        launch_fill_u32(vis, 0u, f.n_paths, c->stream);
-        launch_generate(f, ps, c->stream);
-        for (uint32_t b = 0; b <= f.max_bounces; ++b) {
-            launch_extend(S, ps.q[b & 1u], ps.counters + b, f.n_paths, ps.o, ps.d, ps.hits,
-                          ps.counters + kExtendFetchBase + b, c->ovf.as<uint32_t>(), nullptr, c->stream);
-            launch_shade(S, f, ps, b, c->stream);
-            if (b < f.max_bounces)
-                launch_connect(S, ps.counters + kShadowCounterBase + b, 2u * f.n_paths, ps.sh_o, ps.sh_d, ps.sh_c,
-                               ps.ln0, ps.ln1, ps.counters + kConnectFetchBase + b, c->ovf.as<uint32_t>(), vis, b,
-                               c->stream);
-        }
-        launch_adjoint_init(ps, f.n_paths, lrem, c->stream);
-        // pass 2: replay with the same random numbers, scatter gradients
         const uint32_t nrec = (uint32_t)(f.n_paths * recs_per_path);
         if (deterministic) launch_fill_u32(rec_key, kNoRecord, nrec, c->stream);
-        launch_fill_u32(ps.counters, 0u, kCounterWords, c->stream);
-        launch_generate(f, ps, c->stream);
-        for (uint32_t b = 0; b <= f.max_bounces; ++b) {
-            launch_extend(S, ps.q[b & 1u], ps.counters + b, f.n_paths, ps.o, ps.d, ps.hits,
-                          ps.counters + kExtendFetchBase + b, c->ovf.as<uint32_t>(), nullptr, c->stream);
-            launch_shade_adjoint(S, f, ps, b, vis, lrem, c->adj_dldi.as<float4>(), inv_spp, rec_key, rec_val,
-                                 c->stream);
+        // lanes as in ohpt_render: each runs both passes over its own path range
+        const int nl = f.n_paths >= (1u << 18) ? ohpt_context::kLanes : 1;
+        const uint32_t split = nl == 1 ? f.n_paths : ((f.n_paths / 2 + 63u) & ~63u);
+        if (nl > 1) HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+        for (int l = 0; l < nl; ++l) {
+            const hipStream_t s = nl == 1 ? c->stream : c->lane_stream[l];
+            if (nl > 1) HIPCHK(hipStreamWaitEvent(s, c->ev_fork, 0));
+            FrameParams fl = f;
+            fl.pid_base = l == 0 ? 0u : split;
+            fl.n_paths = l == 0 ? split : f.n_paths - split;
+            const PathState pl = lane_view(ps, fl.pid_base, l);
+            uint32_t* ovf = lane_ovf(c, l);
+            // pass 1: forward, recording shadow visibility
+            launch_fill_u32(pl.counters, 0u, kCounterWords, s);
+            launch_fill_u32(vis + fl.pid_base, 0u, fl.n_paths, s);
+            launch_generate(fl, pl, s);
+            launch_extend(S, pl.q[0], pl.counters, fl.n_paths, pl.o, pl.d, pl.hits, pl.counters + kExtendFetchBase,
+                          ovf, nullptr, s);
+            for (uint32_t b = 0; b <= f.max_bounces; ++b) {
+                launch_shade(S, fl, pl, b, s);
+                if (b < f.max_bounces)
+                    launch_extend_connect(S, pl.q[(b + 1) & 1u], pl.counters + b + 1, fl.n_paths, pl.o, pl.d,
+                                          pl.hits, pl.counters + kShadowCounterBase + b, 2u * fl.n_paths, pl.sh_o,
+                                          pl.sh_d, pl.sh_c, pl.ln0, pl.ln1, pl.counters + kExtendFetchBase + b + 1,
+                                          ovf, vis, b, nullptr, s);
+            }
+            launch_adjoint_init(pl, fl.pid_base, fl.n_paths, lrem, s);
+            // pass 2: replay with the same random numbers, scatter gradients
+            launch_fill_u32(pl.counters, 0u, kCounterWords, s);
+            launch_generate(fl, pl, s);
+            for (uint32_t b = 0; b <= f.max_bounces; ++b) {
+                launch_extend(S, pl.q[b & 1u], pl.counters + b, fl.n_paths, pl.o, pl.d, pl.hits,
+                              pl.counters + kExtendFetchBase + b, ovf, nullptr, s);
+                launch_shade_adjoint(S, fl, pl, b, vis, lrem, c->adj_dldi.as<float4>(), inv_spp, rec_key, rec_val, s);
+            }
+            if (nl > 1) {
+                HIPCHK(hipEventRecord(c->ev_lane[l], s));
+                HIPCHK(hipStreamWaitEvent(c->stream, c->ev_lane[l], 0));
+            }
         }
         if (deterministic) {
             // stable sort of the fixed-slot records by texel, then in-order segment sums

@@ -365,11 +365,12 @@ __global__ __launch_bounds__(256) void k_segment_reduce(const uint32_t* __restri
 }
 
 // Remaining radiance for replay = the path's total radiance from the forward pass.
-__global__ __launch_bounds__(256) void k_adjoint_init(uint32_t n, const float4* __restrict__ L,
+__global__ __launch_bounds__(256) void k_adjoint_init(uint32_t pid_base, uint32_t n, const float4* __restrict__ L,
                                                       const float4* __restrict__ ln0, const float4* __restrict__ ln1,
                                                       float4* __restrict__ lrem) {
-    const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
-    if (pid >= n) return;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t pid = pid_base + i;
     const float4 a = L[pid], b = ln0[pid], c = ln1[pid];
     lrem[pid] = make_float4(a.x + b.x + c.x, a.y + b.y + c.y, a.z + b.z + c.z, 0.0f);
 }
@@ -489,8 +490,8 @@ void launch_segment_reduce(const uint32_t* keys, const uint32_t* vals, uint32_t 
     if (n) k_segment_reduce<<<div_up(n, 256), 256, 0, s>>>(keys, vals, n, arena);
 }
 
-void launch_adjoint_init(const PathState& ps, uint32_t n, float4* lrem, hipStream_t s) {
-    k_adjoint_init<<<div_up(n, 256), 256, 0, s>>>(n, ps.L, ps.ln0, ps.ln1, lrem);
+void launch_adjoint_init(const PathState& ps, uint32_t pid_base, uint32_t n, float4* lrem, hipStream_t s) {
+    if (n) k_adjoint_init<<<div_up(n, 256), 256, 0, s>>>(pid_base, n, ps.L, ps.ln0, ps.ln1, lrem);
 }
 
 void launch_accumulate(const FrameParams& f, const PathState& ps, float4* accum, float4* moments,

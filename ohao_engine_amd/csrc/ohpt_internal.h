@@ -67,11 +67,11 @@ void launch_connect(const DevScene& S, const uint32_t* count, uint32_t max_count
                     const float4* sh_d, const float4* sh_c, float4* ln0, float4* ln1, uint32_t* fetch,
                     uint32_t* ovf, uint32_t* vis, uint32_t bounce, hipStream_t s);
 // One launch for the extension rays of bounce b+1 (closest hit, into `hits`)
-// and the shadow rays of bounce b (any hit, as launch_connect without `vis`).
+// and the shadow rays of bounce b (any hit, as launch_connect).
 void launch_extend_connect(const DevScene& S, const uint32_t* queue, const uint32_t* count_ext, uint32_t max_ext,
                            const float4* ray_o, const float4* ray_d, float4* hits, const uint32_t* count_sh,
                            uint32_t max_sh, const float4* sh_o, const float4* sh_d, const float4* sh_c, float4* ln0,
-                           float4* ln1, uint32_t* fetch, uint32_t* ovf, uint32_t bounce,
+                           float4* ln1, uint32_t* fetch, uint32_t* ovf, uint32_t* vis, uint32_t bounce,
                            unsigned long long* stats, hipStream_t s);
 // Caller rays (ox,oy,oz,tmin,dx,dy,dz,tmax) -> (t, prim|miss, u, v) / occluded flags.
 void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* fetch,
@@ -122,7 +122,8 @@ void launch_shade_adjoint(const DevScene& S, const FrameParams& f, const PathSta
 // Deterministic mode: sums runs of equal keys (sorted records) into arena[key].
 void launch_segment_reduce(const uint32_t* keys, const uint32_t* vals, uint32_t n, float* arena, hipStream_t s);
 // lrem[pid] = total radiance of path pid from the forward pass
-void launch_adjoint_init(const PathState& ps, uint32_t n, float4* lrem, hipStream_t s);
+// lrem[pid] = L + light-NEE + env-NEE radiance for pid in [pid_base, pid_base + n)
+void launch_adjoint_init(const PathState& ps, uint32_t pid_base, uint32_t n, float4* lrem, hipStream_t s);
 void launch_accumulate(const FrameParams& f, const PathState& ps, float4* accum, float4* moments,
                        float4* aov_albedo, float4* aov_normal, hipStream_t s);
 void launch_resolve(const float4* accum, float4* out, uint32_t n, hipStream_t s);

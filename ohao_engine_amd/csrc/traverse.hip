@@ -422,10 +422,10 @@ void launch_connect(const DevScene& S, const uint32_t* count, uint32_t max_count
 void launch_extend_connect(const DevScene& S, const uint32_t* queue, const uint32_t* count_ext, uint32_t max_ext,
                            const float4* ray_o, const float4* ray_d, float4* hits, const uint32_t* count_sh,
                            uint32_t max_sh, const float4* sh_o, const float4* sh_d, const float4* sh_c, float4* ln0,
-                           float4* ln1, uint32_t* fetch, uint32_t* ovf, uint32_t bounce,
+                           float4* ln1, uint32_t* fetch, uint32_t* ovf, uint32_t* vis, uint32_t bounce,
                            unsigned long long* stats, hipStream_t s) {
     if (max_ext + max_sh == 0) return;
-    FusedIO io{ExtendIO{queue, ray_o, ray_d, hits}, ConnectIO{sh_o, sh_d, sh_c, ln0, ln1, nullptr, bounce}, 0u};
+    FusedIO io{ExtendIO{queue, ray_o, ray_d, hits}, ConnectIO{sh_o, sh_d, sh_c, ln0, ln1, vis, bounce}, 0u};
     const uint32_t grid = std::min(trace_grid_blocks(), (max_ext + max_sh + kTraceBlock - 1) / kTraceBlock);
     k_extend_connect<<<grid, kTraceBlock, 0, s>>>(S, io, count_ext, count_sh, fetch, ovf, stats);
 }
