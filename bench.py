@@ -40,9 +40,9 @@ def log(msg):
 
 def trace_bytes(rays, visits, tests):
     """Algorithmic bytes of the traversal launches: 32 B ray in + 16 B result
-    (hit record, or the shadow ray's 16 B contribution) per ray, 64 B per node
-    visit, 48 B per triangle test."""
-    return 48.0 * rays + 64.0 * visits + 48.0 * tests
+    (hit record, or the shadow ray's 16 B contribution) per ray, 128 B per
+    4-wide node visit (one cache line), 48 B per triangle test."""
+    return 48.0 * rays + 128.0 * visits + 48.0 * tests
 
 
 BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak

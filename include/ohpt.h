@@ -170,6 +170,10 @@ int ohpt_scene_tree_depth(ohpt_context* ctx, uint32_t* out);
  * host pointers; sizes in elements as documented in the implementation. */
 int ohpt_scene_export_bvh(ohpt_context* ctx, void* nodes_out /* (n-1)*64 B */,
                           void* tris_out /* n*48 B */, uint32_t* leaf_prims_out /* n */);
+/* The 4-wide nodes every ray query traverses (the binary LBVH collapsed two
+ * levels per node): *count_out = node count; nodes4_out (count*128 B, may be
+ * NULL to query the count) receives them when capacity >= count. */
+int ohpt_scene_export_bvh4(ohpt_context* ctx, void* nodes4_out, uint32_t capacity, uint32_t* count_out);
 
 /* Materials and textures: updateRTMaterialParams (rt_build.cpp:313-389)
  * rewrites materials without touching the BVH. */

@@ -64,7 +64,8 @@ struct ohpt_context {
     bool committed = false;
     uint32_t tree_depth = 0;
 
-    DevBuf d_nodes, d_tris, d_attrs, d_prim, d_slot_of;
+    DevBuf d_nodes, d_nodes4, d_tris, d_attrs, d_prim, d_slot_of;
+    uint32_t n_nodes4 = 0;
 
     std::vector<ohpt_material> h_materials;
     DevBuf d_mats;
@@ -112,7 +113,7 @@ namespace {
 
 DevScene scene_view(ohpt_context* c) {
     DevScene S{};
-    S.nodes = c->d_nodes.as<float4>();
+    S.nodes4 = c->d_nodes4.as<float4>();
     S.tris = c->d_tris.as<float4>();
     S.attrs = c->d_attrs.as<float4>();
     S.n_tris = c->committed ? c->n_tris : 0u;
@@ -490,6 +491,7 @@ int ohpt_scene_commit(ohpt_context* c) {
         c->committed = true;
         c->in_scene = false;
         c->tree_depth = 0;
+        c->n_nodes4 = 0;
         return build_emitters(c);
     }
     DevBuf pos, nrm, uv, mat, scratch;
@@ -525,16 +527,21 @@ int ohpt_scene_commit(ohpt_context* c) {
     b.slot_of = c->d_slot_of.as<uint32_t>();
     b.scratch = scratch.p;
     HIPCHK(hipEventRecord(c->ev_start, c->stream));
-    uint32_t depth = 0;
-    hipError_t e = lbvh_build(b, c->stream, &depth);
+    uint32_t depth = 0, n4 = 0;
+    hipError_t e = lbvh_build(b, c->stream, &depth, &n4);
     if (e != hipSuccess) return fail_hip("lbvh_build", e);
-    HIPCHK(hipEventRecord(c->ev_stop, c->stream));
-    HIPCHK(hipEventSynchronize(c->ev_stop));
-    c->stats.ms_build = elapsed(c->ev_start, c->ev_stop);
     c->tree_depth = depth;
+    c->n_nodes4 = 0;
     if (depth > kMaxTreeDepth)
         return fail(OHPT_ERR_UNSUPPORTED, "ohpt_scene_commit: LBVH depth " + std::to_string(depth) +
                                               " exceeds the traversal stack");
+    HIPCHK(c->d_nodes4.ensure((size_t)n4 * 128));
+    e = bvh4_collapse(b, c->d_nodes4.as<float4>(), c->stream);
+    if (e != hipSuccess) return fail_hip("bvh4_collapse", e);
+    HIPCHK(hipEventRecord(c->ev_stop, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev_stop));
+    c->stats.ms_build = elapsed(c->ev_start, c->ev_stop);
+    c->n_nodes4 = n4;
     c->committed = true;
     c->in_scene = false;
     return build_emitters(c);
@@ -555,6 +562,18 @@ int ohpt_scene_export_bvh(ohpt_context* c, void* nodes_out, void* tris_out, uint
     if (nodes_out) HIPCHK(hipMemcpyAsync(nodes_out, c->d_nodes.p, (size_t)ni * 64, hipMemcpyDeviceToHost, c->stream));
     if (tris_out) HIPCHK(hipMemcpyAsync(tris_out, c->d_tris.p, (size_t)n * 48, hipMemcpyDeviceToHost, c->stream));
     if (leaf_prims_out) HIPCHK(hipMemcpyAsync(leaf_prims_out, c->d_prim.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+int ohpt_scene_export_bvh4(ohpt_context* c, void* nodes4_out, uint32_t capacity, uint32_t* count_out) {
+    if (!c || !c->committed) return fail(OHPT_ERR_NO_SCENE, "ohpt_scene_export_bvh4: no committed scene");
+    if (!count_out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_export_bvh4: null count");
+    HIPCHK(hipSetDevice(c->device));
+    *count_out = c->n_nodes4;
+    if (!nodes4_out || c->n_nodes4 == 0) return OHPT_OK;
+    if (capacity < c->n_nodes4) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_export_bvh4: buffer holds fewer nodes");
+    HIPCHK(hipMemcpyAsync(nodes4_out, c->d_nodes4.p, (size_t)c->n_nodes4 * 128, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return OHPT_OK;
 }

@@ -1,7 +1,8 @@
 // LBVH builder for gfx950: 30-bit Morton codes of triangle centroids, a stable
 // LSD radix sort with LDS-local ranking (8-bit digits, wave64 ballot match),
 // Karras 2012 hierarchy over (morton << 32 | prim) keys, and a bottom-up AABB
-// refit with acquire/release counters. Replaces the driver-built BLAS/TLAS of
+// refit level by level, then a collapse to 4-wide nodes (two binary levels per
+// node) for traversal. Replaces the driver-built BLAS/TLAS of
 // ohao/render/rt/rt_acceleration_structure.cpp:205-309, :419-535.
 //
 // Every floating-point step is IEEE add/mul/div/min/max without contraction,
@@ -333,11 +334,97 @@ __global__ __launch_bounds__(256) void k_refit_level(const float4* __restrict__ 
     reinterpret_cast<uint32_t*>(nd)[14] = node_parent[node];
 }
 
+// --- 4-wide collapse ----------------------------------------------------------
+// Every internal node at even depth becomes a 4-wide node; they are numbered in
+// binary-node order by an exclusive scan of the even-depth flags (tiles of
+// kCollapseTile nodes: per-tile counts, a scan of the counts, per-tile ranks),
+// so the numbering does not depend on scheduling.
+constexpr uint32_t kCollapseItems = 16;
+constexpr uint32_t kCollapseTile = 256 * kCollapseItems;
+
+__global__ __launch_bounds__(256) void k_even_count(const uint32_t* __restrict__ depth, uint32_t ni,
+                                                    uint32_t* __restrict__ tile_count) {
+    __shared__ uint32_t red[256];
+    const uint32_t b0 = blockIdx.x * kCollapseTile + threadIdx.x * kCollapseItems;
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < kCollapseItems; ++k) {
+        const uint32_t i = b0 + k;
+        if (i < ni && (depth[i] & 1u) == 0) ++c;
+    }
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_count[blockIdx.x] = red[0];
+}
+
+// tile_base: exclusive scan of the tile counts
+__global__ __launch_bounds__(256) void k_even_map(const uint32_t* __restrict__ depth, uint32_t ni,
+                                                  const uint32_t* __restrict__ tile_base, uint32_t* __restrict__ map) {
+    __shared__ uint32_t part[256];
+    const uint32_t b0 = blockIdx.x * kCollapseTile + threadIdx.x * kCollapseItems;
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < kCollapseItems; ++k) {
+        const uint32_t i = b0 + k;
+        if (i < ni && (depth[i] & 1u) == 0) ++c;
+    }
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = tile_base[blockIdx.x] + part[threadIdx.x] - c;
+    for (uint32_t k = 0; k < kCollapseItems; ++k) {
+        const uint32_t i = b0 + k;
+        if (i >= ni) break;
+        map[i] = ((depth[i] & 1u) == 0) ? run++ : kMiss;
+    }
+}
+
+// One thread per even-depth binary node: its 4-wide node holds its children,
+// an internal child replaced by that child's two children (boxes as refitted).
+__global__ __launch_bounds__(256) void k_collapse(const float* __restrict__ nodes_f, const uint32_t* __restrict__ depth,
+                                                  const uint32_t* __restrict__ map, uint32_t ni,
+                                                  float4* __restrict__ nodes4) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ni || (depth[i] & 1u)) return;
+    const uint32_t* nodes_u = reinterpret_cast<const uint32_t*>(nodes_f);
+    float b[6][4];
+    uint32_t ref[4] = {kMiss, kMiss, kMiss, kMiss};
+#pragma unroll
+    for (int a = 0; a < 6; ++a) b[a][0] = b[a][1] = b[a][2] = b[a][3] = 0.0f;
+    uint32_t m = 0;
+    for (int c = 0; c < 2; ++c) {
+        const uint32_t ch = nodes_u[(size_t)i * 16 + 12 + c];
+        if (ch & kLeafBit) {
+            for (int a = 0; a < 6; ++a) b[a][m] = nodes_f[(size_t)i * 16 + 6 * c + a];
+            ref[m++] = ch;
+        } else {
+            for (int cc = 0; cc < 2; ++cc) {
+                const uint32_t g = nodes_u[(size_t)ch * 16 + 12 + cc];
+                for (int a = 0; a < 6; ++a) b[a][m] = nodes_f[(size_t)ch * 16 + 6 * cc + a];
+                ref[m++] = (g & kLeafBit) ? g : map[g];
+            }
+        }
+    }
+    float4* q = nodes4 + (size_t)map[i] * 8;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) q[a] = make_float4(b[a][0], b[a][1], b[a][2], b[a][3]);
+    q[6] = make_float4(__uint_as_float(ref[0]), __uint_as_float(ref[1]), __uint_as_float(ref[2]),
+                       __uint_as_float(ref[3]));
+    q[7] = make_float4(__uint_as_float(i), __uint_as_float(m), 0.0f, 0.0f);
+}
+
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 struct ScratchLayout {
     size_t keys0, vals0, keys1, vals1, hist, digit, partial, node_parent, leaf_parent, node_depth, levels,
-        level_nodes, total;
+        level_nodes, map4, tiles4, total;
 };
 ScratchLayout layout(uint32_t n) {
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -356,6 +443,8 @@ ScratchLayout layout(uint32_t n) {
     L.node_depth = o; o += al((size_t)n * 4 + 4);
     L.levels = o; o += al((size_t)kMaxLevels * 4 * 2);
     L.level_nodes = o; o += al((size_t)n * 4 + 4);
+    L.map4 = o; o += al((size_t)n * 4 + 4);
+    L.tiles4 = o; o += al((size_t)div_up(n, kCollapseTile) * 4 + 8);
     L.total = o;
     return L;
 }
@@ -448,7 +537,7 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t
 
 size_t lbvh_scratch_bytes(uint32_t n) { return layout(n).total; }
 
-hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out) {
+hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out, uint32_t* n4_out) {
     const uint32_t n = b.n;
     if (n == 0) return hipSuccess;
     const ScratchLayout L = layout(n);
@@ -497,11 +586,12 @@ hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out) 
     hipError_t e = hipMemcpyAsync(counts, level_count, sizeof(counts), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
-    uint32_t offs[kMaxLevels], maxd = 0, acc = 0;
+    uint32_t offs[kMaxLevels], maxd = 0, acc = 0, n4 = 0;
     for (uint32_t d = 0; d < kMaxLevels; ++d) {
         offs[d] = acc;
         acc += counts[d];
         if (counts[d]) maxd = d;
+        if ((d & 1u) == 0) n4 += counts[d];
     }
     e = hipMemcpyAsync(level_cursor, offs, sizeof(offs), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
@@ -511,11 +601,31 @@ hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out) 
         k_refit_level<<<div_up(counts[d], 256), 256, 0, s>>>(b.tris, reinterpret_cast<float*>(b.nodes), node_parent,
                                                               level_nodes + offs[d], counts[d]);
     }
+    // 4-wide numbering: even-depth flags -> per-tile counts -> scan -> ranks
+    const uint32_t nt4 = div_up(ni, kCollapseTile);
+    uint32_t* tiles4 = reinterpret_cast<uint32_t*>(sc + L.tiles4);
+    uint32_t* map4 = reinterpret_cast<uint32_t*>(sc + L.map4);
+    k_even_count<<<nt4, 256, 0, s>>>(node_depth, ni, tiles4);
+    k_scan_digit<<<1, 256, 0, s>>>(tiles4, nt4, tiles4 + nt4);
+    k_even_map<<<nt4, 256, 0, s>>>(node_depth, ni, tiles4, map4);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     // deepest leaf = two levels below the deepest internal node (root at 0, leaves counted)
     if (depth_out) *depth_out = maxd + 2;
+    if (n4_out) *n4_out = n4;
     return hipStreamSynchronize(s);
+}
+
+hipError_t bvh4_collapse(const LbvhBuffers& b, float4* nodes4, hipStream_t s) {
+    const uint32_t n = b.n;
+    if (n == 0) return hipSuccess;
+    const ScratchLayout L = layout(n);
+    const char* sc = reinterpret_cast<const char*>(b.scratch);
+    const uint32_t ni = n > 1 ? n - 1 : 1;
+    k_collapse<<<div_up(ni, 256), 256, 0, s>>>(reinterpret_cast<const float*>(b.nodes),
+                                                reinterpret_cast<const uint32_t*>(sc + L.node_depth),
+                                                reinterpret_cast<const uint32_t*>(sc + L.map4), ni, nodes4);
+    return hipGetLastError();
 }
 
 }  // namespace ohpt

@@ -3,6 +3,10 @@
 // Layouts in HBM (all arrays in LBVH leaf order, "slot" = sorted position):
 //   nodes  : (n-1) x 64 B  {c0.lo.xyz, c0.hi.xyz, c1.lo.xyz, c1.hi.xyz, ref0, ref1, parent, pad}
 //            ref bit31 set = leaf (triangle slot), else internal node index
+//            (binary LBVH: build and refit only)
+//   nodes4 : n4 x 128 B    {lo.x[4]}, {lo.y[4]}, {lo.z[4]}, {hi.x[4]}, {hi.y[4]}, {hi.z[4]}, {ref[4]},
+//                          {binary node, used slots, 0, 0}; ref bit31 = leaf, 0xFFFFFFFF = empty
+//            slot, else 4-wide node index (traversed by every ray query)
 //   tris   : n x 48 B      {v0.xyz, prim}, {v1.xyz, 0}, {v2.xyz, 0}
 //   attrs  : n x 64 B      {n0.xyz, n1.x}, {n1.yz, n2.xy}, {n2.z, uv0.xy, uv1.x}, {uv1.y, uv2.xy, mat}
 //   mats   : m x 64 B      {base.rgb, rough}, {metal, f0, emis.rg}, {emis.b, albedo_tex, rough_tex, metal_tex}, {0}
@@ -150,7 +154,7 @@ constexpr uint32_t kGradRecordsPerVertex = 20;   // 4 texels x (3 albedo + rough
 constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
 
 struct DevScene {
-    const float4* nodes;
+    const float4* nodes4;
     const float4* tris;
     const float4* attrs;
     uint32_t n_tris;

@@ -26,8 +26,12 @@ struct LbvhBuffers {
     void* scratch;
 };
 size_t lbvh_scratch_bytes(uint32_t n);
-// Returns 0 or a HIP error; writes max leaf depth to *depth_out (host) when not null.
-hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out);
+// Returns 0 or a HIP error; writes max leaf depth to *depth_out and the number
+// of 4-wide nodes to *n4_out (host) when not null.
+hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out, uint32_t* n4_out);
+// After lbvh_build with the same buffers (scratch still holding depths and the
+// 4-wide numbering): writes the n4 x 128 B 4-wide nodes.
+hipError_t bvh4_collapse(const LbvhBuffers& b, float4* nodes4, hipStream_t s);
 
 // Standalone stable radix sort of 32-bit keys with 32-bit values (exposed for tests).
 size_t radix_sort_scratch_bytes(uint32_t n);
@@ -44,10 +48,11 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t
 #endif
 constexpr uint32_t kTraceBlock = OHPT_TRACE_BLOCK;
 constexpr uint32_t kStackLds = OHPT_STACK_LDS;   // per-thread LDS stack entries
-constexpr uint32_t kStackOvf = 48;               // per-thread global overflow entries
-constexpr uint32_t kMaxTreeDepth = 60;   // commit rejects deeper trees
-// at most one push per tree level: registers (4) + LDS + overflow must cover the deepest tree
-static_assert(4 + kStackLds + kStackOvf >= kMaxTreeDepth, "traversal stack shallower than the deepest tree");
+constexpr uint32_t kStackOvf = 64;               // per-thread global overflow entries
+constexpr uint32_t kMaxTreeDepth = 60;   // commit rejects deeper trees (binary leaf depth, root = 1)
+// at most three pushes per 4-wide level, and a binary tree of leaf depth D has
+// at most D/2 levels of 4-wide nodes: registers (4) + LDS + overflow cover that
+static_assert(4 + kStackLds + kStackOvf >= 3 * (kMaxTreeDepth / 2), "traversal stack shallower than the deepest tree");
 static_assert(kTraceBlock % 64 == 0 && kTraceBlock <= 1024, "trace block must be whole waves");
 
 // Persistent-thread grid of the traversal kernels and the size of their
@@ -108,9 +113,8 @@ struct PathState {
     uint32_t capacity;
 };
 constexpr uint32_t kShadowCounterBase = 64;       // [64 + b]: shadow rays queued at bounce b
-constexpr uint32_t kExtendFetchBase = 128;        // [128 + b]: dynamic-fetch counter of extend b
-constexpr uint32_t kConnectFetchBase = 192;       // [192 + b]: dynamic-fetch counter of connect b
-constexpr uint32_t kCounterWords = 256;
+constexpr uint32_t kExtendFetchBase = 128;        // [128 + b]: dynamic-fetch counter of traversal launch b
+constexpr uint32_t kCounterWords = 192;
 
 void launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_generate(const FrameParams& f, const PathState& ps, hipStream_t s);

@@ -6,13 +6,17 @@
 // fetching, Aila & Laine 2009) — so lanes do not wait for the longest ray of a
 // batch and no blocks are launched for rays that do not exist.
 //
-// Short-stack traversal of the 64-byte two-child LBVH nodes: the top four
-// stack entries live in VGPRs (shifted on push/pop), deeper entries spill to a
-// per-thread column of LDS ([depth][thread], bank-conflict free), and beyond
-// that to a per-thread slot in global memory. Leaves (one triangle each) are
-// intersected as soon as their box is hit, near child first; the visiting
-// order and every FP operation match the sequential restatement in the
-// oracle, so hits agree bit for bit.
+// Short-stack traversal of the 128-byte 4-wide nodes (the binary LBVH
+// collapsed two levels per node, halving the chain of dependent node fetches
+// per ray; one node is one 128-byte cache line): the top four stack entries
+// live in VGPRs (shifted on push/pop), deeper entries spill to a per-thread
+// column of LDS ([depth][thread], bank-conflict free), and beyond that to a
+// per-thread slot in global memory. At each node the hit children are ordered
+// by entry distance (order_key); hit leaves (one triangle each) are intersected in
+// that order, then the internal children whose entry is still within the
+// closest hit continue, nearest first. The visiting order and every FP
+// operation match the sequential restatement in the oracle, so hits agree bit
+// for bit.
 //
 // Ray/box: robust slab test with tmax scaled by 1+2*gamma(3) (Ize 2013).
 // Ray/triangle: watertight test of Woop, Benthin & Wald (JCGT 2013) with the
@@ -106,6 +110,21 @@ __device__ __forceinline__ bool tri_hit(const RayPre& r, float4 a, float4 b, flo
     uout = V * rcp;
     vout = W * rcp;
     return true;
+}
+
+// Child order key: the entry distance's bits mapped to an unsigned total
+// order, the two low bits replaced by the slot (keys of one node are distinct;
+// entries within four ulps order by slot). A missed child (+inf) sorts last.
+constexpr uint32_t kKeyMiss = 0xFF800000u;
+__device__ __forceinline__ uint32_t order_key(float e, uint32_t slot) {
+    const uint32_t b = __float_as_uint(e);
+    const uint32_t m = (b & 0x80000000u) ? 0xFFFFFFFFu : 0x80000000u;
+    return ((b ^ m) & ~3u) | slot;
+}
+__device__ __forceinline__ void cx(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = min(a, b), hi = max(a, b);
+    a = lo;
+    b = hi;
 }
 
 // Register/LDS/global short stack.
@@ -285,22 +304,39 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
         do {
             if (active) {
                 ++nv;
-                const float4* np = S.nodes + (size_t)cur * 4;
-                const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
-                const float e0 = box_hit(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, tmin, tclose);
-                const float e1 = box_hit(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tmin, tclose);
-                const uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
-                const bool h0 = e0 != INFINITY, h1 = e1 != INFINITY;
-                const bool sw = h1 && (!h0 || e1 < e0);
-                const uint32_t A = sw ? c1 : c0, B = sw ? c0 : c1;
-                const bool hA = sw ? h1 : h0, hB = sw ? h0 : h1;
-                const bool lA = hA && (A & kLeafBit), lB = hB && (B & kLeafBit);
+                const float4* np = S.nodes4 + (size_t)cur * 8;
+                const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+                const uint4 rf = reinterpret_cast<const uint4*>(np)[6];
+                // all four slabs unconditionally (empty slots masked afterwards), then
+                // order the slots by sort key with a 5-comparator min/max network
+                float e0 = box_hit(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tmin, tclose);
+                float e1 = box_hit(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tmin, tclose);
+                float e2 = box_hit(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tmin, tclose);
+                float e3 = box_hit(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, tmin, tclose);
+                e0 = rf.x == kMiss ? INFINITY : e0;
+                e1 = rf.y == kMiss ? INFINITY : e1;
+                e2 = rf.z == kMiss ? INFINITY : e2;
+                e3 = rf.w == kMiss ? INFINITY : e3;
+                uint32_t key[4] = {order_key(e0, 0), order_key(e1, 1), order_key(e2, 2), order_key(e3, 3)};
+                cx(key[0], key[1]);
+                cx(key[2], key[3]);
+                cx(key[0], key[2]);
+                cx(key[1], key[3]);
+                cx(key[1], key[2]);
+                uint32_t c[4];
+                float e[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t sl = key[q] & 3u;
+                    c[q] = sl == 0 ? rf.x : (sl == 1 ? rf.y : (sl == 2 ? rf.z : rf.w));
+                    e[q] = key[q] >= kKeyMiss ? INFINITY : (sl == 0 ? e0 : (sl == 1 ? e1 : (sl == 2 ? e2 : e3)));
+                }
                 bool done = false;
 #pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    const bool test = (k == 0 ? lA : lB) && !done;
+                for (int k = 0; k < 4; ++k) {
+                    const bool test = e[k] != INFINITY && (c[k] & kLeafBit) && !done;
                     if (test) {
-                        const uint32_t s = (k == 0 ? A : B) & ~kLeafBit;
+                        const uint32_t s = c[k] & ~kLeafBit;
                         const float4* tp = S.tris + (size_t)s * 3;
                         const float4 ta = tp[0], tb = tp[1], tc = tp[2];
                         float tt, uu, vv;
@@ -313,16 +349,19 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
                     }
                 }
                 if (!done) {
-                    const bool iA = hA && !(A & kLeafBit), iB = hB && !(B & kLeafBit);
-                    if (iA) {
-                        cur = A;
-                        if (iB) st.push(B, lds_col, ovf);
-                    } else if (iB) {
-                        cur = B;
-                    } else if (st.empty()) {
-                        done = true;
+                    // internal children still within the closest hit: nearest next, the rest pushed farthest first
+                    bool go[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) go[k] = e[k] != INFINITY && !(c[k] & kLeafBit) && e[k] <= tclose;
+                    const int f = go[0] ? 0 : (go[1] ? 1 : (go[2] ? 2 : (go[3] ? 3 : 4)));
+                    if (f == 4) {
+                        if (st.empty()) done = true;
+                        else cur = st.pop(lds_col, ovf);
                     } else {
-                        cur = st.pop(lds_col, ovf);
+#pragma unroll
+                        for (int k = 3; k >= 1; --k)
+                            if (go[k] && k > f) st.push(c[k], lds_col, ovf);
+                        cur = f == 0 ? c[0] : (f == 1 ? c[1] : (f == 2 ? c[2] : c[3]));
                     }
                 }
                 if (done) {

@@ -72,6 +72,7 @@ SIGNATURES = [
     ("ohpt_scene_triangle_count", C.c_int, [_vp, C.POINTER(C.c_uint64)]),
     ("ohpt_scene_tree_depth", C.c_int, [_vp, _u32p]),
     ("ohpt_scene_export_bvh", C.c_int, [_vp, _vp, _vp, _u32p]),
+    ("ohpt_scene_export_bvh4", C.c_int, [_vp, _vp, C.c_uint32, _u32p]),
     ("ohpt_upload_materials", C.c_int, [_vp, _vp, C.c_uint32]),
     ("ohpt_upload_texture", C.c_int, [_vp, C.POINTER(C.c_int32), _f32p, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("ohpt_update_texture", C.c_int, [_vp, C.c_int32, _f32p, C.c_int]),
@@ -242,6 +243,16 @@ class Context:
         _check(self.L.ohpt_scene_export_bvh(self.h, nodes.ctypes.data, tris.ctypes.data,
                                             prim.ctypes.data_as(_u32p)), "ohpt_scene_export_bvh")
         return nodes, tris, prim
+
+    def export_bvh4(self):
+        """The 4-wide traversal nodes as an (n4, 32) uint32 array (128 B per node)."""
+        cnt = C.c_uint32(0)
+        _check(self.L.ohpt_scene_export_bvh4(self.h, None, 0, C.byref(cnt)), "ohpt_scene_export_bvh4")
+        nodes = np.zeros((cnt.value, 32), np.uint32)
+        if cnt.value:
+            _check(self.L.ohpt_scene_export_bvh4(self.h, nodes.ctypes.data, cnt.value, C.byref(cnt)),
+                   "ohpt_scene_export_bvh4")
+        return nodes
 
     def set_materials(self, mats):
         m = np.ascontiguousarray(mats, S.MATERIAL_DTYPE)

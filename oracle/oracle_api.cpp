@@ -126,6 +126,12 @@ uint32_t oracle_scene_bvh(void* h, void* nodes_out, float* tris_out, uint32_t* p
     return (uint32_t)b.nodes.size();
 }
 
+uint32_t oracle_scene_bvh4(void* h, void* nodes4_out) {
+    const Bvh& b = static_cast<Scene*>(h)->bvh;
+    if (nodes4_out) std::memcpy(nodes4_out, b.nodes4.data(), b.nodes4.size() * sizeof(Bvh4Node));
+    return (uint32_t)b.nodes4.size();
+}
+
 // rays: n x 8 (ox,oy,oz,tmin,dx,dy,dz,tmax); hits: n x 4 (t, prim bits, u, v)
 void oracle_trace_batch(void* h, const float* rays, uint32_t n, float* hits, uint32_t* visits,
                         uint32_t* tests) {

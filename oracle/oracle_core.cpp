@@ -214,6 +214,7 @@ void lbvh_build(const float* tp, uint32_t n, Bvh& out) {
             for (int a = 0; a < 3; ++a) { nd.b[c * 6 + a] = lo[a]; nd.b[c * 6 + 3 + a] = hi[a]; }
         }
     }
+    bvh4_collapse(out);
 }
 
 // ------------------------------------------------------------------ FP32 traversal
@@ -302,38 +303,109 @@ inline bool tri_hit(const RayPre& r, const float* v0, const float* v1, const flo
 
 }  // namespace
 
-// Traversal order (identical in the HIP extend/connect kernels): at each
-// internal node both child boxes are tested against [tmin, t_closest]; the hit
-// children are ordered near-first by entry distance (child 0 first on ties);
-// hit leaf children are intersected immediately in that order; then the near
-// internal child is visited next and the far one pushed on the stack.
+// ---------------------------------------------------------------- 4-wide collapse
+void bvh4_collapse(Bvh& bvh) {
+    bvh.nodes4.clear();
+    if (bvh.n == 0) return;
+    const size_t ni = bvh.nodes.size();
+    std::vector<uint32_t> depth(ni, 0), map(ni, kEmpty);
+    {
+        std::vector<uint32_t> st{0u};
+        while (!st.empty()) {
+            const uint32_t i = st.back();
+            st.pop_back();
+            for (int c = 0; c < 2; ++c) {
+                const uint32_t ch = bvh.nodes[i].child[c];
+                if (!(ch & kLeafBit)) { depth[ch] = depth[i] + 1; st.push_back(ch); }
+            }
+        }
+    }
+    uint32_t cnt = 0;
+    for (size_t i = 0; i < ni; ++i)
+        if ((depth[i] & 1u) == 0) map[i] = cnt++;
+    bvh.nodes4.assign(cnt, Bvh4Node{});
+    for (size_t i = 0; i < ni; ++i) {
+        if (map[i] == kEmpty) continue;
+        Bvh4Node& q = bvh.nodes4[map[i]];
+        for (int k = 0; k < 4; ++k) {
+            q.lox[k] = q.loy[k] = q.loz[k] = 0.0f;
+            q.hix[k] = q.hiy[k] = q.hiz[k] = 0.0f;
+            q.child[k] = kEmpty;
+        }
+        q.bin_node = (uint32_t)i;
+        q.pad[0] = q.pad[1] = 0;
+        uint32_t m = 0;
+        auto put = [&](const float* b, uint32_t ref) {
+            q.lox[m] = b[0]; q.loy[m] = b[1]; q.loz[m] = b[2];
+            q.hix[m] = b[3]; q.hiy[m] = b[4]; q.hiz[m] = b[5];
+            q.child[m] = ref;
+            ++m;
+        };
+        const BvhNode& nd = bvh.nodes[i];
+        for (int c = 0; c < 2; ++c) {
+            const uint32_t ch = nd.child[c];
+            if (ch & kLeafBit) {
+                put(&nd.b[6 * c], ch);
+            } else {
+                const BvhNode& a = bvh.nodes[ch];
+                for (int cc = 0; cc < 2; ++cc) {
+                    const uint32_t g = a.child[cc];
+                    put(&a.b[6 * cc], (g & kLeafBit) ? g : map[g]);
+                }
+            }
+        }
+        q.count = m;
+    }
+}
+
+// Child order key: the entry distance's bits mapped to an unsigned total
+// order with the two low bits replaced by the slot, so the keys of one node
+// are distinct and entries within four ulps order by slot; +inf (missed or
+// empty) sorts last.
+static inline uint32_t order_key(float e, uint32_t slot) {
+    uint32_t b;
+    std::memcpy(&b, &e, 4);
+    const uint32_t m = (b & 0x80000000u) ? 0xFFFFFFFFu : 0x80000000u;
+    return ((b ^ m) & ~3u) | slot;
+}
+
+// Traversal order (identical in the HIP traversal kernels): at each 4-wide
+// node the used child boxes are tested against [tmin, t_closest] and the hit
+// children ordered by order_key; hit leaves are intersected in that order;
+// internal children whose entry distance is still within the closest hit
+// continue, the nearest next and the others pushed farthest first.
 template <bool kAnyHit>
-static HitF32 traverse(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax,
-                       uint32_t* node_visits, uint32_t* tri_tests, bool* occ) {
+static HitF32 traverse4(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax,
+                        uint32_t* node_visits, uint32_t* tri_tests, bool* occ) {
     HitF32 h{tmax, 0xFFFFFFFFu, 0.0f, 0.0f};
     uint32_t nv = 0, nt = 0;
     if (occ) *occ = false;
     if (bvh.n == 0) return h;
     RayPre r;
     ray_setup(o, d, r);
-    uint32_t stack[128];
+    uint32_t stack[256];
     int sp = 0;
     uint32_t cur = 0;
     float tclose = tmax;
     for (;;) {
         ++nv;
-        const BvhNode& nd = bvh.nodes[cur];
-        const float e0 = box_hit(r, &nd.b[0], &nd.b[3], tmin, tclose);
-        const float e1 = box_hit(r, &nd.b[6], &nd.b[9], tmin, tclose);
-        const bool h0 = e0 != INFINITY, h1 = e1 != INFINITY;
-        const bool sw = h1 && (!h0 || e1 < e0);
-        const uint32_t A = sw ? nd.child[1] : nd.child[0], B = sw ? nd.child[0] : nd.child[1];
-        const bool hA = sw ? h1 : h0, hB = sw ? h0 : h1;
-        const uint32_t leaves[2] = {A, B};
-        const bool hl[2] = {hA && (A & kLeafBit) != 0, hB && (B & kLeafBit) != 0};
-        for (int k = 0; k < 2; ++k) {
-            if (!hl[k]) continue;
-            const float* t = &bvh.tri[(size_t)(leaves[k] & ~kLeafBit) * 12];
+        const Bvh4Node& nd = bvh.nodes4[cur];
+        float e[4];
+        for (int k = 0; k < 4; ++k) {
+            if (nd.child[k] == kEmpty) { e[k] = INFINITY; continue; }
+            const float lo[3] = {nd.lox[k], nd.loy[k], nd.loz[k]}, hi[3] = {nd.hix[k], nd.hiy[k], nd.hiz[k]};
+            e[k] = box_hit(r, lo, hi, tmin, tclose);
+        }
+        uint32_t key[4];
+        for (int k = 0; k < 4; ++k) key[k] = order_key(e[k], (uint32_t)k);
+        std::sort(key, key + 4);
+        int ord[4];
+        for (int j = 0; j < 4; ++j) ord[j] = (int)(key[j] & 3u);
+        for (int j = 0; j < 4; ++j) {
+            const int k = ord[j];
+            if (e[k] == INFINITY) break;
+            if (!(nd.child[k] & kLeafBit)) continue;
+            const float* t = &bvh.tri[(size_t)(nd.child[k] & ~kLeafBit) * 12];
             float tt, uu, vv;
             ++nt;
             if (tri_hit(r, t, t + 4, t + 8, tmin, tclose, tt, uu, vv)) {
@@ -348,17 +420,19 @@ static HitF32 traverse(const Bvh& bvh, const float o[3], const float d[3], float
                 }
             }
         }
-        const bool iA = hA && !(A & kLeafBit), iB = hB && !(B & kLeafBit);
-        if (iA && iB) {
-            cur = A;
-            stack[sp++] = B;
-        } else if (iA) {
-            cur = A;
-        } else if (iB) {
-            cur = B;
-        } else {
+        uint32_t go[4];
+        int ng = 0;
+        for (int j = 0; j < 4; ++j) {
+            const int k = ord[j];
+            if (e[k] == INFINITY) break;
+            if (!(nd.child[k] & kLeafBit) && e[k] <= tclose) go[ng++] = nd.child[k];
+        }
+        if (ng == 0) {
             if (sp == 0) break;
             cur = stack[--sp];
+        } else {
+            cur = go[0];
+            for (int j = ng - 1; j >= 1; --j) stack[sp++] = go[j];
         }
     }
     if (node_visits) *node_visits = nv;
@@ -368,12 +442,12 @@ static HitF32 traverse(const Bvh& bvh, const float o[3], const float d[3], float
 
 HitF32 trace_f32(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax,
                  uint32_t* node_visits, uint32_t* tri_tests) {
-    return traverse<false>(bvh, o, d, tmin, tmax, node_visits, tri_tests, nullptr);
+    return traverse4<false>(bvh, o, d, tmin, tmax, node_visits, tri_tests, nullptr);
 }
 
 bool occluded_f32(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax) {
     bool occ = false;
-    traverse<true>(bvh, o, d, tmin, tmax, nullptr, nullptr, &occ);
+    traverse4<true>(bvh, o, d, tmin, tmax, nullptr, nullptr, &occ);
     return occ;
 }
 

@@ -164,19 +164,36 @@ struct BvhNode {            // 64 B, identical to the device layout
 static_assert(sizeof(BvhNode) == 64, "node layout");
 constexpr uint32_t kLeafBit = 0x80000000u;
 
+// 4-wide node (128 B), identical to the device layout: the binary LBVH
+// collapsed two levels at a time.
+struct Bvh4Node {
+    float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];   // child boxes
+    uint32_t child[4];      // bit31 set = leaf (sorted triangle slot); kEmpty = unused slot
+    uint32_t bin_node;      // binary internal node this node was collapsed from
+    uint32_t count;         // used slots
+    uint32_t pad[2];
+};
+static_assert(sizeof(Bvh4Node) == 128, "node4 layout");
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
 struct Bvh {
     uint32_t n = 0;
     std::vector<BvhNode> nodes;      // n-1 internal nodes (1 when n == 1)
+    std::vector<Bvh4Node> nodes4;    // collapsed tree traversed by trace/occlusion queries
     std::vector<float> tri;          // sorted triangles, 12 floats each (v0 xyz, prim, v1 xyz, 0, v2 xyz, 0)
     std::vector<uint32_t> prim;      // sorted slot -> original triangle id
     std::vector<uint64_t> keys;      // morton << 32 | prim, sorted
 };
 
 uint32_t morton30(float cx, float cy, float cz, const float cmin[3], const float cmax[3]);
-void lbvh_build(const float* tri_pos /* n*9 */, uint32_t n, Bvh& out);
+void lbvh_build(const float* tri_pos /* n*9 */, uint32_t n, Bvh& out);   // binary LBVH + 4-wide collapse
+// Collapse (called by lbvh_build): every internal node at even depth becomes a
+// 4-wide node, numbered in binary-node order; its slots are its binary
+// children in order, an internal child replaced by that child's two children.
+void bvh4_collapse(Bvh& bvh);
 
 struct HitF32 { float t; uint32_t prim; float u, v; };
-// FP32 closest hit / occlusion through the LBVH, op-for-op as the HIP extend/connect kernels.
+// FP32 closest hit / occlusion through the 4-wide tree, op-for-op as the HIP traversal kernels.
 HitF32 trace_f32(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax,
                  uint32_t* node_visits, uint32_t* tri_tests);
 bool occluded_f32(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax);

@@ -60,6 +60,8 @@ def lib():
         L.oracle_scene_rebuild_emitters.argtypes = [vp]
         L.oracle_scene_bvh.argtypes = [vp, vp, f32p, u32p, u64p]
         L.oracle_scene_bvh.restype = C.c_uint32
+        L.oracle_scene_bvh4.argtypes = [vp, vp]
+        L.oracle_scene_bvh4.restype = C.c_uint32
         L.oracle_trace_batch.argtypes = [vp, f32p, C.c_uint32, f32p, u32p, u32p]
         L.oracle_occluded_batch.argtypes = [vp, f32p, C.c_uint32, u32p]
         L.oracle_brute_batch.argtypes = [vp, f32p, C.c_uint32, C.c_double, f64p, u32p, u32p, u32p, C.c_uint32]
@@ -155,6 +157,14 @@ class OracleScene:
         self.L.oracle_scene_bvh(self.h, nodes.ctypes.data, _p(tris), _p(prim, C.c_uint32),
                                 _p(keys, C.c_uint64))
         return nodes, tris, prim, keys
+
+    def bvh4(self):
+        """4-wide nodes as (n4, 32) uint32: lo.x[4] lo.y[4] lo.z[4] hi.x[4] hi.y[4] hi.z[4] ref[4] meta[4]."""
+        n4 = self.L.oracle_scene_bvh4(self.h, None)
+        out = np.zeros((n4, 32), np.uint32)
+        if n4:
+            self.L.oracle_scene_bvh4(self.h, out.ctypes.data)
+        return out
 
     def trace(self, rays):
         rays = np.ascontiguousarray(rays, np.float32)

@@ -55,6 +55,7 @@ def test_lbvh_bit_exact(oracle, name):
     # parent field (14) is not part of the oracle's layout contract for the root; compare boxes + children
     assert np.array_equal(g_nodes[:, :14], c_nodes[:, :14]) if sc.num_triangles > 1 else \
         np.array_equal(g_nodes[:, :14], c_nodes[:, :14])
+    assert np.array_equal(ctx.export_bvh4(), o.bvh4()), "4-wide nodes differ from the oracle's collapse"
     assert 1 <= ctx.tree_depth() <= 60
     ctx.close()
 

@@ -1,5 +1,6 @@
 """Oracle LBVH + traversal pins: Morton closed form, Karras invariants
-(contiguous leaf ranges, parents' boxes contain children), and the FP32
+(contiguous leaf ranges, parents' boxes contain children), the 4-wide
+collapse (every leaf once, boxes nest), and the FP32 4-wide
 traversal against an FP64 brute force over all triangles (closest primitive
 equal except t-ties within 1e-5 relative, which are listed)."""
 import numpy as np
@@ -88,6 +89,63 @@ def _check_tree(nodes, tris, prim, keys, pos):
     assert (l, h) == (0, n - 1)
 
 
+EMPTY = 0xFFFFFFFF
+
+
+def _check_bvh4(nodes4, nodes, n):
+    """4-wide collapse: one node per even-depth binary node, numbered in binary
+    order; every leaf and every non-root node referenced once; slot boxes equal
+    the binary child boxes, and an internal slot's box is the union of its
+    node's slot boxes."""
+    child = nodes[:, 12:14]
+    boxes = nodes[:, :12].view(np.float32)
+    ni = nodes.shape[0]
+    depth = np.zeros(ni, np.int64)
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        for c in child[i]:
+            if c < LEAF:
+                depth[c] = depth[i] + 1
+                stack.append(int(c))
+    even = np.flatnonzero(depth % 2 == 0)
+    assert nodes4.shape[0] == even.size
+    assert np.array_equal(nodes4[:, 28], even)                 # binary node of each 4-wide node, in order
+    index_of = {int(b): q for q, b in enumerate(even)}
+    lo = np.stack([nodes4[:, 0:4], nodes4[:, 4:8], nodes4[:, 8:12]], -1).view(np.float32)
+    hi = np.stack([nodes4[:, 12:16], nodes4[:, 16:20], nodes4[:, 20:24]], -1).view(np.float32)
+    refs = nodes4[:, 24:28]
+    used = refs != EMPTY
+    assert np.array_equal(nodes4[:, 29], used.sum(1))
+    assert np.all(used[:, :2]) and np.all(used.sum(1) >= 2)
+    assert np.all(used[:, :-1] | ~used[:, 1:])                # used slots first
+    leaves = refs[used & (refs >= LEAF)] - LEAF
+    inner = refs[used & (refs < LEAF)]
+    if n > 1:
+        assert sorted(leaves.tolist()) == list(range(n))
+    assert sorted(inner.tolist()) == list(range(1, nodes4.shape[0]))
+    for q, b in enumerate(even):
+        want_box, want_ref = [], []
+        for k in range(2):
+            c = int(child[b, k])
+            if c >= LEAF:
+                want_box.append(boxes[b, 6 * k:6 * k + 6]); want_ref.append(c)
+            else:
+                for kk in range(2):
+                    g = int(child[c, kk])
+                    want_box.append(boxes[c, 6 * kk:6 * kk + 6])
+                    want_ref.append(g if g >= LEAF else index_of[g])
+        m = len(want_ref)
+        assert refs[q, :m].tolist() == want_ref
+        got = np.concatenate([lo[q, :m], hi[q, :m]], 1)
+        assert np.array_equal(got.view(np.uint32), np.array(want_box).view(np.uint32))
+        for k in range(m):
+            r = int(refs[q, k])
+            if r < LEAF:
+                cu = used[r]
+                assert np.array_equal(lo[q, k], lo[r][cu].min(0)) and np.array_equal(hi[q, k], hi[r][cu].max(0))
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 17, 1000])
 def test_lbvh_invariants(oracle, n):
     pos = random_soup(n, seed=n)
@@ -96,6 +154,7 @@ def test_lbvh_invariants(oracle, n):
     o = oracle.OracleScene(sc)
     nodes, tris, prim, keys = o.bvh()
     _check_tree(nodes, tris, prim, keys, pos)
+    _check_bvh4(o.bvh4(), nodes, n)
 
 
 def test_lbvh_duplicate_centroids(oracle):
@@ -107,6 +166,7 @@ def test_lbvh_duplicate_centroids(oracle):
     o = oracle.OracleScene(sc)
     nodes, tris, prim, keys = o.bvh()
     _check_tree(nodes, tris, prim, keys, pos)
+    _check_bvh4(o.bvh4(), nodes, 64)
 
 
 def random_rays(n, seed, origin_box=6.0, tmin=0.0):
