@@ -44,7 +44,7 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t
 #define OHPT_TRACE_BLOCK 128
 #endif
 #ifndef OHPT_STACK_LDS
-#define OHPT_STACK_LDS 32
+#define OHPT_STACK_LDS 24   // 12 KB per block: 10 blocks (VGPR-bound 5 waves/SIMD) fit a CU's 160 KB with room to spare
 #endif
 constexpr uint32_t kTraceBlock = OHPT_TRACE_BLOCK;
 constexpr uint32_t kStackLds = OHPT_STACK_LDS;   // per-thread LDS stack entries

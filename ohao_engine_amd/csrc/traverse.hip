@@ -19,8 +19,9 @@
 // for bit.
 //
 // Ray/box: robust slab test with tmax scaled by 1+2*gamma(3) (Ize 2013).
-// Ray/triangle: watertight test of Woop, Benthin & Wald (JCGT 2013) with the
-// double-precision fallback on zero edge functions.
+// Ray/triangle: watertight test of Woop, Benthin & Wald (JCGT 2013); an edge
+// function that rounds to zero is re-evaluated with an exact-sign difference
+// of products (fused multiply-adds) instead of the paper's double precision.
 #pragma clang fp contract(off)
 
 #include "ohpt_internal.h"
@@ -78,6 +79,18 @@ __device__ __forceinline__ float box_hit(const RayPre& r, float lx, float ly, fl
     return (t0 <= t1) ? t0 : INFINITY;
 }
 
+// a*b - c*d by Kahan's algorithm with fused multiply-adds (error-free
+// product residual; relative error below 2 ulp, Jeannerod, Louvet & Muller
+// 2013): exactly zero only when the exact value is, otherwise of the exact
+// sign -- what the watertight test needs when a float edge function rounds
+// to zero.
+__device__ __forceinline__ float diff_of_products(float a, float b, float c, float d) {
+    const float w = c * d;
+    const float e = fmaf(-c, d, w);
+    const float f = fmaf(a, b, -w);
+    return f + e;
+}
+
 __device__ __forceinline__ bool tri_hit(const RayPre& r, float4 a, float4 b, float4 c, float tmin, float tmax,
                                         float& tout, float& uout, float& vout) {
     const float Ax0 = a.x - r.ox, Ay0 = a.y - r.oy, Az0 = a.z - r.oz;
@@ -94,9 +107,9 @@ __device__ __forceinline__ bool tri_hit(const RayPre& r, float4 a, float4 b, flo
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+        U = diff_of_products(Cx, By, Cy, Bx);
+        V = diff_of_products(Ax, Cy, Ay, Cx);
+        W = diff_of_products(Bx, Ay, By, Ax);
     }
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = U + V + W;
@@ -241,7 +254,9 @@ struct FusedIO {
     }
 };
 
-template <int kMode, class IO>
+// kStats: count node visits and triangle tests into stats[0..1] (profiling
+// launches only; the counters cost registers).
+template <int kMode, bool kStats, class IO>
 __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io, uint32_t count,
                                                  uint32_t* __restrict__ fetch, uint32_t* lds_col, uint32_t* ovf,
                                                  unsigned long long* stats) {
@@ -251,8 +266,9 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
     st.r0 = st.r1 = st.r2 = st.r3 = 0;
     st.reset();
     uint32_t cur = 0, j = 0;
-    float tmin = 0.0f, tclose = 0.0f;
-    float4 hit = make_float4(0, 0, 0, 0);
+    // closest hit so far: distance tclose (tmax while none), slot hprim, barycentrics hu, hv
+    float tmin = 0.0f, tclose = 0.0f, hu = 0.0f, hv = 0.0f;
+    uint32_t hprim = kMiss;
     bool active = false, anyhit = kMode == 1;
     uint32_t nv = 0, nt = 0;
     // The first ray of every thread is its global index (no atomics: launches
@@ -268,11 +284,12 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
         io.load(j, o, d, tmin, tmax);
         r = ray_setup(o, d);
         tclose = tmax;
-        hit = make_float4(tmax, __uint_as_float(kMiss), 0.0f, 0.0f);
+        hprim = kMiss;
+        hu = hv = 0.0f;
         cur = 0;
         st.reset();
         active = S.n_tris > 0;
-        if (!active) io.store(S, j, hit);
+        if (!active) io.store(S, j, make_float4(tclose, __uint_as_float(hprim), hu, hv));
     };
     {
         const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -303,7 +320,7 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
         // ---- traverse until too many lanes have finished (or to the end once drained)
         do {
             if (active) {
-                ++nv;
+                if constexpr (kStats) ++nv;
                 const float4* np = S.nodes4 + (size_t)cur * 8;
                 const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
                 const uint4 rf = reinterpret_cast<const uint4*>(np)[6];
@@ -340,10 +357,12 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
                         const float4* tp = S.tris + (size_t)s * 3;
                         const float4 ta = tp[0], tb = tp[1], tc = tp[2];
                         float tt, uu, vv;
-                        ++nt;
+                        if constexpr (kStats) ++nt;
                         if (tri_hit(r, ta, tb, tc, tmin, tclose, tt, uu, vv)) {
                             tclose = tt;
-                            hit = make_float4(tt, __uint_as_float(s), uu, vv);
+                            hprim = s;
+                            hu = uu;
+                            hv = vv;
                             if (anyhit) done = true;
                         }
                     }
@@ -365,13 +384,13 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
                     }
                 }
                 if (done) {
-                    io.store(S, j, hit);
+                    io.store(S, j, make_float4(tclose, __uint_as_float(hprim), hu, hv));
                     active = false;
                 }
             }
         } while (__popcll(__ballot(active)) >= (drained ? 1 : kRefillThreshold));
     }
-    if (stats) {
+    if constexpr (kStats) {
         for (int off = 32; off > 0; off >>= 1) {
             nv += __shfl_xor(nv, off);
             nt += __shfl_xor(nt, off);
@@ -383,21 +402,23 @@ __device__ __forceinline__ void persistent_trace(const DevScene& S, const IO& io
     }
 }
 
+template <bool kStats>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(DevScene S, ExtendIO io, const uint32_t* __restrict__ count,
                                                         uint32_t* __restrict__ fetch, uint32_t* __restrict__ ovf,
                                                         unsigned long long* __restrict__ stats) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
-    persistent_trace<0>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, stats);
+    persistent_trace<0, kStats>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, stats);
 }
 
 __global__ __launch_bounds__(kTraceBlock) void k_connect(DevScene S, ConnectIO io, const uint32_t* __restrict__ count,
                                                          uint32_t* __restrict__ fetch, uint32_t* __restrict__ ovf) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
-    persistent_trace<1>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
+    persistent_trace<1, false>(S, io, *count, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
 }
 
+template <bool kStats>
 __global__ __launch_bounds__(kTraceBlock) void k_extend_connect(DevScene S, FusedIO io,
                                                                 const uint32_t* __restrict__ count_ext,
                                                                 const uint32_t* __restrict__ count_sh,
@@ -407,8 +428,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend_connect(DevScene S, Fuse
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
     io.n_ext = *count_ext;
-    persistent_trace<2>(S, io, io.n_ext + *count_sh, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf,
-                        stats);
+    persistent_trace<2, kStats>(S, io, io.n_ext + *count_sh, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf,
+                                stats);
 }
 
 template <bool kAnyHit>
@@ -416,7 +437,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_rays(DevScene S, RaysIO io, uin
                                                       uint32_t* __restrict__ ovf) {
     __shared__ uint32_t lds[kStackLds * kTraceBlock];
     const uint32_t g = blockIdx.x * kTraceBlock + threadIdx.x;
-    persistent_trace<kAnyHit ? 1 : 0>(S, io, n, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
+    persistent_trace<kAnyHit ? 1 : 0, false>(S, io, n, fetch, lds + threadIdx.x, ovf + (size_t)g * kStackOvf, nullptr);
 }
 
 __global__ void k_zero(uint32_t* p) { *p = 0u; }
@@ -430,7 +451,7 @@ uint32_t trace_grid_blocks() {
         int dev = 0, cus = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend, kTraceBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_extend_connect<false>, kTraceBlock, 0);
         if (cus <= 0) cus = 256;
         if (per_cu <= 0) per_cu = 8;
         g_grid = (uint32_t)(cus * per_cu);
@@ -446,7 +467,8 @@ void launch_extend(const DevScene& S, const uint32_t* queue, const uint32_t* cou
     if (max_count == 0) return;
     ExtendIO io{queue, ray_o, ray_d, hits};
     const uint32_t grid = std::min(trace_grid_blocks(), (max_count + kTraceBlock - 1) / kTraceBlock);
-    k_extend<<<grid, kTraceBlock, 0, s>>>(S, io, count, fetch, ovf, stats);
+    if (stats) k_extend<true><<<grid, kTraceBlock, 0, s>>>(S, io, count, fetch, ovf, stats);
+    else k_extend<false><<<grid, kTraceBlock, 0, s>>>(S, io, count, fetch, ovf, nullptr);
 }
 
 void launch_connect(const DevScene& S, const uint32_t* count, uint32_t max_count, const float4* sh_o,
@@ -466,7 +488,8 @@ void launch_extend_connect(const DevScene& S, const uint32_t* queue, const uint3
     if (max_ext + max_sh == 0) return;
     FusedIO io{ExtendIO{queue, ray_o, ray_d, hits}, ConnectIO{sh_o, sh_d, sh_c, ln0, ln1, vis, bounce}, 0u};
     const uint32_t grid = std::min(trace_grid_blocks(), (max_ext + max_sh + kTraceBlock - 1) / kTraceBlock);
-    k_extend_connect<<<grid, kTraceBlock, 0, s>>>(S, io, count_ext, count_sh, fetch, ovf, stats);
+    if (stats) k_extend_connect<true><<<grid, kTraceBlock, 0, s>>>(S, io, count_ext, count_sh, fetch, ovf, stats);
+    else k_extend_connect<false><<<grid, kTraceBlock, 0, s>>>(S, io, count_ext, count_sh, fetch, ovf, nullptr);
 }
 
 void launch_trace_rays(const DevScene& S, const float4* rays, float4* hits, uint32_t n, uint32_t* fetch,

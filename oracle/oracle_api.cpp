@@ -126,6 +126,11 @@ uint32_t oracle_scene_bvh(void* h, void* nodes_out, float* tris_out, uint32_t* p
     return (uint32_t)b.nodes.size();
 }
 
+// ray8: (ox,oy,oz,tmin,dx,dy,dz,tmax); tri9: v0,v1,v2; tuv3 written on a hit
+int oracle_tri_hit(const float* ray8, const float* tri9, float* tuv3) {
+    return tri_hit_f32(ray8, ray8 + 4, tri9, ray8[3], ray8[7], tuv3) ? 1 : 0;
+}
+
 uint32_t oracle_scene_bvh4(void* h, void* nodes4_out) {
     const Bvh& b = static_cast<Scene*>(h)->bvh;
     if (nodes4_out) std::memcpy(nodes4_out, b.nodes4.data(), b.nodes4.size() * sizeof(Bvh4Node));

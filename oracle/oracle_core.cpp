@@ -219,8 +219,9 @@ void lbvh_build(const float* tp, uint32_t n, Bvh& out) {
 
 // ------------------------------------------------------------------ FP32 traversal
 // Robust slab test (tmax scaled by 1 + 2*gamma(3), Ize 2013 / pbrt-v3) and the
-// watertight ray/triangle test of Woop, Benthin, Wald (JCGT 2013) with the
-// double-precision fallback for zero edge functions.
+// watertight ray/triangle test of Woop, Benthin, Wald (JCGT 2013); edge
+// functions that round to zero are re-evaluated with the exact-sign
+// difference of products below (the paper re-evaluates in double precision).
 namespace {
 
 struct RayPre {
@@ -267,6 +268,18 @@ inline float box_hit(const RayPre& r, const float* lo, const float* hi, float tm
 
 inline float comp(const float* v, int k) { return v[k]; }
 
+// a*b - c*d by Kahan's algorithm with fused multiply-adds (error-free
+// product residual; relative error below 2 ulp, Jeannerod, Louvet & Muller
+// 2013): exactly zero only when the exact value is, otherwise of the exact
+// sign -- what the watertight test needs when a float edge function rounds
+// to zero.
+inline float diff_of_products(float a, float b, float c, float d) {
+    const float w = c * d;
+    const float e = std::fma(-c, d, w);
+    const float f = std::fma(a, b, -w);
+    return f + e;
+}
+
 // returns true and updates t,u,v on a hit with tmin < t < tmax
 inline bool tri_hit(const RayPre& r, const float* v0, const float* v1, const float* v2,
                     float tmin, float tmax, float& t_out, float& u_out, float& v_out) {
@@ -283,9 +296,9 @@ inline bool tri_hit(const RayPre& r, const float* v0, const float* v1, const flo
     float V = Ax * Cy - Ay * Cx;
     float W = Bx * Ay - By * Ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+        U = diff_of_products(Cx, By, Cy, Bx);
+        V = diff_of_products(Ax, Cy, Ay, Cx);
+        W = diff_of_products(Bx, Ay, By, Ax);
     }
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     float det = U + V + W;
@@ -438,6 +451,12 @@ static HitF32 traverse4(const Bvh& bvh, const float o[3], const float d[3], floa
     if (node_visits) *node_visits = nv;
     if (tri_tests) *tri_tests = nt;
     return h;
+}
+
+bool tri_hit_f32(const float o[3], const float d[3], const float v[9], float tmin, float tmax, float tuv[3]) {
+    RayPre r;
+    ray_setup(o, d, r);
+    return tri_hit(r, v, v + 3, v + 6, tmin, tmax, tuv[0], tuv[1], tuv[2]);
 }
 
 HitF32 trace_f32(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax,

@@ -194,6 +194,8 @@ void bvh4_collapse(Bvh& bvh);
 
 struct HitF32 { float t; uint32_t prim; float u, v; };
 // FP32 closest hit / occlusion through the 4-wide tree, op-for-op as the HIP traversal kernels.
+// One ray against one triangle (v: 3 vertices x xyz): the same FP32 watertight test.
+bool tri_hit_f32(const float o[3], const float d[3], const float v[9], float tmin, float tmax, float tuv[3]);
 HitF32 trace_f32(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax,
                  uint32_t* node_visits, uint32_t* tri_tests);
 bool occluded_f32(const Bvh& bvh, const float o[3], const float d[3], float tmin, float tmax);

@@ -60,6 +60,8 @@ def lib():
         L.oracle_scene_rebuild_emitters.argtypes = [vp]
         L.oracle_scene_bvh.argtypes = [vp, vp, f32p, u32p, u64p]
         L.oracle_scene_bvh.restype = C.c_uint32
+        L.oracle_tri_hit.argtypes = [f32p, f32p, f32p]
+        L.oracle_tri_hit.restype = C.c_int
         L.oracle_scene_bvh4.argtypes = [vp, vp]
         L.oracle_scene_bvh4.restype = C.c_uint32
         L.oracle_trace_batch.argtypes = [vp, f32p, C.c_uint32, f32p, u32p, u32p]

@@ -104,6 +104,26 @@ def test_trace_bit_exact_and_brute_force(oracle, name):
     ctx.close()
 
 
+def test_watertight_rounded_zero_edges(oracle):
+    """Rays whose float edge function rounds to zero although the exact value
+    does not: the GPU decides by the exact sign (rational reference) and agrees
+    with the oracle bit for bit."""
+    from watertight_cases import scene_and_rays
+    pos, rays, want = scene_and_rays(128, seed=7)
+    log("watertight", pos.shape[0], "cases,", int(want.sum()), "expected hits")
+    sc = plain_scene(pos)
+    ctx = ohpt.Context(8, 8)
+    ctx.set_geometry(sc.pos, sc.nrm, sc.uv, sc.mat)
+    g = ctx.trace_rays(rays)
+    o = oracle.OracleScene(sc)
+    c, _, _ = o.trace(rays)
+    assert np.array_equal(g.view(np.uint32), c.view(np.uint32))
+    got = g[:, 1].view(np.uint32) != 0xFFFFFFFF
+    assert np.array_equal(got, want)
+    assert np.all(g[got, 1].view(np.uint32) == np.flatnonzero(got))   # each ray hits its own triangle
+    ctx.close()
+
+
 def test_trace_tmin_tmax_window(oracle):
     sc = SCENES["soup1000"]()
     rays = random_rays(5000, 21)
