@@ -194,6 +194,11 @@ int ohpt_upload_env(ohpt_context* ctx, const float* rgba, uint32_t width, uint32
 int ohpt_set_env_intensity(ohpt_context* ctx, float scale);   /* setEnvIntensityScale */
 int ohpt_export_env_cdf(ohpt_context* ctx, float* marginal /* H */, float* conditional /* W*H */,
                         float* integral);
+/* Environment importance samples as the integrator draws them
+ * (shaders/includes/rt/env_sampling.glsl:16-74), for inspection: u = n pairs in
+ * [0,1)^2 (host), out = n x (dir.x, dir.y, dir.z, pdf) (host); pdf 0 where no
+ * sample is produced. */
+int ohpt_env_sample(ohpt_context* ctx, const float* u, float* out, uint32_t n);
 
 /* ---- per-frame state --------------------------------------------------- */
 int ohpt_set_camera(ohpt_context* ctx, const ohpt_camera* cam);

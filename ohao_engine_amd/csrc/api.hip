@@ -77,7 +77,7 @@ struct ohpt_context {
     uint32_t n_emit = 0;
     float emit_total = 0.0f;
 
-    DevBuf d_env, d_marg, d_cond, d_rowsum, d_integral;
+    DevBuf d_env, d_marg, d_cond, d_rowsum, d_integral, d_marg_guide, d_cond_guide;
     uint32_t env_w = 0, env_h = 0;
     float env_intensity = 1.0f;
 
@@ -103,7 +103,10 @@ struct ohpt_context {
     std::vector<hipEvent_t> ev_pool;
     // A frame's paths split into kLanes independent wavefronts on their own
     // streams, so the tail of one lane's traversal overlaps the other's work.
-    static constexpr int kLanes = 2;
+#ifndef OHPT_LANES
+#define OHPT_LANES 2
+#endif
+    static constexpr int kLanes = OHPT_LANES;
     hipStream_t lane_stream[kLanes] = {};
     hipEvent_t ev_fork = nullptr, ev_lane[kLanes] = {};
     ohpt_stats stats{};
@@ -130,6 +133,10 @@ DevScene scene_view(ohpt_context* c) {
     S.env = c->d_env.as<float4>();
     S.env_marg = c->d_marg.as<float>();
     S.env_cond = c->d_cond.as<float>();
+    S.env_marg_guide = c->d_marg_guide.as<uint32_t>();
+    S.env_cond_guide = c->d_cond_guide.as<uint32_t>();
+    S.env_marg_log2 = env_guide_log2(c->env_h);
+    S.env_cond_log2 = env_guide_log2(c->env_w);
     S.env_w = c->env_w;
     S.env_h = c->env_h;
     S.env_intensity = c->env_intensity;
@@ -673,9 +680,12 @@ int ohpt_upload_env(ohpt_context* c, const float* rgba, uint32_t w, uint32_t h) 
     HIPCHK(c->d_marg.ensure((size_t)h * 4));
     HIPCHK(c->d_rowsum.ensure((size_t)h * 4));
     HIPCHK(c->d_integral.ensure(4));
+    HIPCHK(c->d_marg_guide.ensure(((size_t(1) << env_guide_log2(h)) + 1) * 4));
+    HIPCHK(c->d_cond_guide.ensure(((size_t(1) << env_guide_log2(w)) + 1) * h * 4));
     HIPCHK(hipMemcpyAsync(c->d_env.p, rgba, n * 16, hipMemcpyHostToDevice, c->stream));
     hipError_t e = env_cdf_build(c->d_env.as<float4>(), w, h, c->d_marg.as<float>(), c->d_cond.as<float>(),
-                                 c->d_rowsum.as<float>(), c->d_integral.as<float>(), c->stream);
+                                 c->d_rowsum.as<float>(), c->d_integral.as<float>(), c->d_marg_guide.as<uint32_t>(),
+                                 c->d_cond_guide.as<uint32_t>(), c->stream);
     if (e != hipSuccess) return fail_hip("env_cdf_build", e);
     HIPCHK(hipStreamSynchronize(c->stream));
     c->env_w = w;
@@ -697,6 +707,22 @@ int ohpt_export_env_cdf(ohpt_context* c, float* marginal, float* conditional, fl
         HIPCHK(hipMemcpyAsync(conditional, c->d_cond.p, (size_t)c->env_w * c->env_h * 4, hipMemcpyDeviceToHost,
                               c->stream));
     if (integral) HIPCHK(hipMemcpyAsync(integral, c->d_integral.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
+int ohpt_env_sample(ohpt_context* c, const float* u, float* out, uint32_t n) {
+    if (!c || (n && (!u || !out))) return fail(OHPT_ERR_INVALID_ARG, "ohpt_env_sample: null argument");
+    if (c->env_w == 0) return fail(OHPT_ERR_STATE, "ohpt_env_sample: no environment");
+    if (n == 0) return OHPT_OK;
+    HIPCHK(hipSetDevice(c->device));
+    DevBuf du, dout;
+    HIPCHK(du.ensure((size_t)n * 8));
+    HIPCHK(dout.ensure((size_t)n * 16));
+    HIPCHK(hipMemcpyAsync(du.p, u, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+    launch_env_sample(scene_view(c), du.as<float>(), dout.as<float4>(), n, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dout.p, (size_t)n * 16, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return OHPT_OK;
 }
@@ -791,18 +817,18 @@ int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
         f.flags = c->st.flags;
         f.firefly_clamp = c->st.firefly_clamp;
         f.ray_eps = c->st.ray_eps;
-        // Lanes: two independent wavefronts on their own streams once the batch
-        // is large enough; one lane on the context stream when profiling, so
-        // the per-kernel events time each kernel alone.
+        // Lanes: kLanes independent wavefronts on their own streams once the
+        // batch is large enough; one lane on the context stream when profiling,
+        // so the per-kernel events time each kernel alone.
         const int nl = (!profile && f.n_paths >= (1u << 18)) ? ohpt_context::kLanes : 1;
-        const uint32_t split = nl == 1 ? f.n_paths : ((f.n_paths / 2 + 63u) & ~63u);
+        const uint32_t part = ((f.n_paths + nl - 1) / nl + 63u) & ~63u;
         if (nl > 1) HIPCHK(hipEventRecord(c->ev_fork, c->stream));
         for (int l = 0; l < nl; ++l) {
             const hipStream_t s = nl == 1 ? c->stream : c->lane_stream[l];
             if (nl > 1) HIPCHK(hipStreamWaitEvent(s, c->ev_fork, 0));
             FrameParams fl = f;
-            fl.pid_base = l == 0 ? 0u : split;
-            fl.n_paths = l == 0 ? split : f.n_paths - split;
+            fl.pid_base = std::min<uint32_t>(f.n_paths, (uint32_t)l * part);
+            fl.n_paths = std::min<uint32_t>(f.n_paths - fl.pid_base, part);
             const PathState pl = lane_view(ps, fl.pid_base, l);
             uint32_t* ovf = lane_ovf(c, l);
             launch_fill_u32(pl.counters, 0u, kCounterWords, s);
@@ -991,14 +1017,14 @@ int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_
         if (deterministic) launch_fill_u32(rec_key, kNoRecord, nrec, c->stream);
         // lanes as in ohpt_render: each runs both passes over its own path range
         const int nl = f.n_paths >= (1u << 18) ? ohpt_context::kLanes : 1;
-        const uint32_t split = nl == 1 ? f.n_paths : ((f.n_paths / 2 + 63u) & ~63u);
+        const uint32_t part = ((f.n_paths + nl - 1) / nl + 63u) & ~63u;
         if (nl > 1) HIPCHK(hipEventRecord(c->ev_fork, c->stream));
         for (int l = 0; l < nl; ++l) {
             const hipStream_t s = nl == 1 ? c->stream : c->lane_stream[l];
             if (nl > 1) HIPCHK(hipStreamWaitEvent(s, c->ev_fork, 0));
             FrameParams fl = f;
-            fl.pid_base = l == 0 ? 0u : split;
-            fl.n_paths = l == 0 ? split : f.n_paths - split;
+            fl.pid_base = std::min<uint32_t>(f.n_paths, (uint32_t)l * part);
+            fl.n_paths = std::min<uint32_t>(f.n_paths - fl.pid_base, part);
             const PathState pl = lane_view(ps, fl.pid_base, l);
             uint32_t* ovf = lane_ovf(c, l);
             // pass 1: forward, recording shadow visibility

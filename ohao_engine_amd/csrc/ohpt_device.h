@@ -171,6 +171,9 @@ struct DevScene {
     const float4* env;             // env_w*env_h RGBA32F, row 0 = +Y
     const float* env_marg;
     const float* env_cond;
+    const uint32_t* env_marg_guide;   // 2^env_marg_log2 + 1 entries
+    const uint32_t* env_cond_guide;   // env_h rows of 2^env_cond_log2 + 1 entries
+    uint32_t env_marg_log2, env_cond_log2;
     uint32_t env_w, env_h;
     float env_intensity;
     uint32_t pad0;
@@ -274,11 +277,25 @@ OHPT_DEV uint32_t cdf_lower_bound(const float* cdf, uint32_t n, float u) {
     return (uint32_t)lo;
 }
 
+// cdf_lower_bound(cdf, n, u) for u in [0, 1), searching only the guide cell of u
+OHPT_DEV uint32_t cdf_lower_bound_guided(const float* cdf, const uint32_t* guide, uint32_t log2m, float u) {
+    const uint32_t m = 1u << log2m;
+    const float kf = ldexpf(u, (int)log2m);   // exact: scaling by a power of two
+    const uint32_t k = kf > 0.0f ? min((uint32_t)kf, m - 1u) : 0u;
+    uint32_t lo = guide[k], hi = guide[k + 1];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] < u) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 OHPT_DEV bool env_sample(const DevScene& S, float u1, float u2, f3& dir, float& pdf) {
     const uint32_t W = S.env_w, H = S.env_h;
-    const uint32_t y = cdf_lower_bound(S.env_marg, H, u1);
+    const uint32_t y = cdf_lower_bound_guided(S.env_marg, S.env_marg_guide, S.env_marg_log2, u1);
     const float* row = S.env_cond + (size_t)y * W;
-    const uint32_t x = cdf_lower_bound(row, W, u2);
+    const uint32_t x = cdf_lower_bound_guided(
+        row, S.env_cond_guide + (size_t)y * ((1u << S.env_cond_log2) + 1u), S.env_cond_log2, u2);
     const float m0 = y > 0 ? S.env_marg[y - 1] : 0.0f, m1 = S.env_marg[y];
     const float c0 = x > 0 ? row[x - 1] : 0.0f, c1 = row[x];
     const float dm = m1 - m0, dc = c1 - c0;

@@ -137,8 +137,14 @@ void launch_resolve_aovs(const float4* accum, const float4* aov_albedo, const fl
 
 // ---- environment (env.hip) -------------------------------------------------
 // Per-row luminance*sin(theta) prefix sums, normalised, plus marginal (env_cdf.cpp:13-61).
+// Also fills the guide tables: marg_guide 2^env_guide_log2(H)+1 entries,
+// cond_guide H rows of 2^env_guide_log2(W)+1 entries.
+uint32_t env_guide_log2(uint32_t n);
 hipError_t env_cdf_build(const float4* env, uint32_t W, uint32_t H, float* marg, float* cond,
-                         float* row_sum, float* integral_dev, hipStream_t s);
+                         float* row_sum, float* integral_dev, uint32_t* marg_guide, uint32_t* cond_guide,
+                         hipStream_t s);
+// Environment importance samples for inspection: u (n x 2) -> out (n x 4: dir.xyz, pdf; pdf 0 = no sample).
+void launch_env_sample(const DevScene& S, const float* u, float4* out, uint32_t n, hipStream_t s);
 
 // ---- denoiser (denoiser.hip) ---------------------------------------------------
 // Inference on device-resident guides (RGBA32F radiance/albedo/normal, F32

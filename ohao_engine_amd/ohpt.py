@@ -81,6 +81,7 @@ SIGNATURES = [
     ("ohpt_upload_env", C.c_int, [_vp, _f32p, C.c_uint32, C.c_uint32]),
     ("ohpt_set_env_intensity", C.c_int, [_vp, C.c_float]),
     ("ohpt_export_env_cdf", C.c_int, [_vp, _f32p, _f32p, _f32p]),
+    ("ohpt_env_sample", C.c_int, [_vp, _f32p, _f32p, C.c_uint32]),
     ("ohpt_set_camera", C.c_int, [_vp, C.POINTER(Camera)]),
     ("ohpt_set_settings", C.c_int, [_vp, C.POINTER(Settings)]),
     ("ohpt_get_settings", C.c_int, [_vp, C.POINTER(Settings)]),
@@ -298,6 +299,13 @@ class Context:
         i = np.zeros(1, np.float32)
         _check(self.L.ohpt_export_env_cdf(self.h, _fp(m), _fp(c), _fp(i)), "ohpt_export_env_cdf")
         return m, c, float(i[0])
+
+    def env_sample(self, u):
+        """Environment importance samples for u (n x 2 in [0,1)): (n x 4) dir.xyz, pdf."""
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+        out = np.zeros((u.shape[0], 4), np.float32)
+        _check(self.L.ohpt_env_sample(self.h, _fp(u), _fp(out), u.shape[0]), "ohpt_env_sample")
+        return out
 
     def set_camera(self, cam: "S.Camera"):
         c = Camera()

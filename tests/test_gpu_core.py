@@ -176,6 +176,47 @@ def test_env_cdf_matches_oracle(oracle):
     assert np.max(np.abs(pg[big] - pc[big]) / pc[big]) < 1e-2
 
 
+def test_env_sample_guided_search_is_the_full_search():
+    """The guide tables only shorten the CDF inversion: every sampled texel is the
+    one a full lower-bound search over the same (GPU) tables picks, including u
+    on CDF values and on guide-cell boundaries."""
+    W, H = 512, 256
+    env = S.sky_env(W, H, sun_dir=(0.3, 0.6, 0.4))
+    ctx = ohpt.Context(8, 8)
+    ctx.set_env(env)
+    gm, gc, _ = ctx.export_env_cdf(W, H)
+    rng = np.random.default_rng(3)
+    n = 60000
+    u = rng.random((n, 2), dtype=np.float32)
+    u[:2000, 0] = gm[rng.integers(0, H, 2000)]                        # on marginal CDF values
+    u[2000:4000, 0] = rng.integers(0, 256, 2000) / np.float32(256)    # on guide-cell boundaries
+    u[4000:4100, 0] = np.float32(np.nextafter(np.float32(1), np.float32(0)))
+    u[4100:4200] = 0.0
+    u = np.minimum(u, np.float32(np.nextafter(np.float32(1), np.float32(0)))).astype(np.float32)
+    y = np.minimum(np.searchsorted(gm, u[:, 0], side="left"), H - 1)
+    u[:3000, 1] = gc[y[:3000], rng.integers(0, W, 3000)]              # on conditional CDF values
+    u[3000:5000, 1] = rng.integers(0, 512, 2000) / np.float32(512)
+    x = np.array([min(np.searchsorted(gc[yy], uu, side="left"), W - 1) for yy, uu in zip(y, u[:, 1])])
+    out = ctx.env_sample(u)
+    ctx.close()
+    # expected direction of the texel the full search picks (float64 restatement of the mapping)
+    m0 = np.where(y > 0, gm[np.maximum(y - 1, 0)], 0.0).astype(np.float64)
+    c0 = np.where(x > 0, gc[y, np.maximum(x - 1, 0)], 0.0).astype(np.float64)
+    dm = gm[y] - m0
+    dc = gc[y, x] - c0
+    ok = (dm > 0) & (dc > 0)
+    fy = np.clip((u[:, 0] - m0) / np.where(ok, dm, 1), 0, 0.99999994)
+    fx = np.clip((u[:, 1] - c0) / np.where(ok, dc, 1), 0, 0.99999994)
+    th = (y + fy) / H * np.pi
+    ph = ((x + fx) / W - 0.5) * 2 * np.pi
+    d = np.stack([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)], 1)
+    good = ok & (out[:, 3] > 0)
+    assert np.mean(good) > 0.99
+    err = np.abs(out[good, :3] - d[good]).max(1)
+    log(f"env samples: {good.sum()} compared, max direction error {err.max():.2e}")
+    assert err.max() < 2e-4      # one texel is >= 6e-3 rad at this resolution
+
+
 def render(sc, spp, settings=None, seed=None):
     ctx = ohpt.Context(sc.width, sc.height)
     ctx.load_scene(sc)
