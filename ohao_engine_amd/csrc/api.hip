@@ -93,7 +93,7 @@ struct ohpt_context {
     DevBuf ps_o, ps_d, ps_T, ps_L, ps_ln0, ps_ln1, ps_aov0, ps_aov1, ps_q0, ps_q1, ps_hits, sh_o, sh_d, sh_c,
         counters, ovf, tmp, tstats;
     uint32_t ovf_rays = 0;
-    DevBuf adj_vis, adj_lrem, adj_dldi;     // path-replay adjoint state
+    DevBuf adj_vis, adj_lrem, adj_dldi, adj_hits;   // path-replay adjoint state
     DevBuf grad_arena;                      // dL/d(texel) of all textures, packed like the textures
     std::vector<size_t> grad_off;           // per-texture offset in the arena (floats)
     DevBuf rec_key, rec_val, rec_scratch;   // deterministic-mode gradient records
@@ -952,10 +952,11 @@ int ohpt_read_texture_gradient(ohpt_context* c, int32_t id, float* out, int on_d
 
 // Path-replay adjoint (Vicini, Speierer, Jakob 2021) of the image made of the
 // first `spp` samples after a reset (sample indices seed .. seed+spp-1):
-// pass 1 renders each batch forward and records shadow visibility per path;
-// pass 2 regenerates the same paths, re-traces them with the same random
-// numbers and scatters dL/d(texel) into the texture gradient buffers
-// (accumulating: call ohpt_zero_gradients first).
+// pass 1 renders each batch forward and records shadow visibility and the
+// closest hit of every vertex per path (16 B per vertex in HBM); pass 2
+// regenerates the same paths with the same random numbers, reads the logged
+// hits instead of re-tracing, and scatters dL/d(texel) into the texture
+// gradient buffers (accumulating: call ohpt_zero_gradients first).
 int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_t spp, uint32_t flags) {
     if (!c || !dl_dimage) return fail(OHPT_ERR_INVALID_ARG, "ohpt_adjoint: null argument");
     if (!c->committed) return fail(OHPT_ERR_NO_SCENE, "ohpt_adjoint: no committed scene");
@@ -982,6 +983,7 @@ int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_
     if (r != OHPT_OK) return r;
     HIPCHK(c->adj_vis.ensure((size_t)c->capacity * 4));
     HIPCHK(c->adj_lrem.ensure((size_t)c->capacity * 16));
+    HIPCHK(c->adj_hits.ensure((size_t)c->capacity * (c->st.max_bounces + 1) * 16));
     uint32_t* rec_key = nullptr;
     uint32_t* rec_val = nullptr;
     if (deterministic) {
@@ -993,7 +995,8 @@ int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_
         rec_val = c->rec_val.as<uint32_t>();
     }
     const DevScene S = scene_view(c);
-    const PathState ps = path_state(c);
+    PathState ps = path_state(c);
+    ps.hit_log = c->adj_hits.as<float4>();
     uint32_t* vis = c->adj_vis.as<uint32_t>();
     float4* lrem = c->adj_lrem.as<float4>();
     const float inv_spp = 1.0f / (float)spp;
@@ -1042,14 +1045,11 @@ int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_
                                           ovf, vis, b, nullptr, s);
             }
             launch_adjoint_init(pl, fl.pid_base, fl.n_paths, lrem, s);
-            // pass 2: replay with the same random numbers, scatter gradients
+            // pass 2: replay with the same random numbers and the logged hits, scatter gradients
             launch_fill_u32(pl.counters, 0u, kCounterWords, s);
             launch_generate(fl, pl, s);
-            for (uint32_t b = 0; b <= f.max_bounces; ++b) {
-                launch_extend(S, pl.q[b & 1u], pl.counters + b, fl.n_paths, pl.o, pl.d, pl.hits,
-                              pl.counters + kExtendFetchBase + b, ovf, nullptr, s);
+            for (uint32_t b = 0; b <= f.max_bounces; ++b)
                 launch_shade_adjoint(S, fl, pl, b, vis, lrem, c->adj_dldi.as<float4>(), inv_spp, rec_key, rec_val, s);
-            }
             if (nl > 1) {
                 HIPCHK(hipEventRecord(c->ev_lane[l], s));
                 HIPCHK(hipStreamWaitEvent(c->stream, c->ev_lane[l], 0));

@@ -136,7 +136,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
     float4 shl_o, shl_d, shl_c, she_o, she_d, she_c;
     if (active) {
         pid = qin[j];
-        const float4 hit = ps.hits[j];
+        float4* const logged = ps.hit_log ? ps.hit_log + (size_t)bounce * ps.capacity + pid : nullptr;
+        const float4 hit = (kAdjoint && logged) ? *logged : ps.hits[j];
+        if (!kAdjoint && logged) *logged = hit;
         const float4 dd = ps.d[pid];
         const f3 d = xyz(dd);
         const float prev_pdf = dd.w;

@@ -111,6 +111,10 @@ struct PathState {
     float4* sh_c;
     uint32_t* counters;   // kCounterWords: extend counts, shadow counts, fetch counters
     uint32_t capacity;
+    // Adjoint only (null otherwise): closest hit of every path vertex, written by the
+    // forward pass at [bounce * capacity + pid] and read back by the replay, which
+    // then traces nothing.
+    float4* hit_log;
 };
 constexpr uint32_t kShadowCounterBase = 64;       // [64 + b]: shadow rays queued at bounce b
 constexpr uint32_t kExtendFetchBase = 128;        // [128 + b]: dynamic-fetch counter of traversal launch b
