@@ -94,6 +94,8 @@ struct ohpt_context {
         counters, ovf, tmp, tstats;
     uint32_t ovf_rays = 0;
     DevBuf adj_vis, adj_lrem, adj_dldi, adj_hits;   // path-replay adjoint state
+    DevBuf rec_key2, rec_val2, rec_count, rec_tiles; // deterministic mode: compacted records
+    size_t grad_total = 0;                           // floats in the gradient arena (record keys are below it)
     DevBuf grad_arena;                      // dL/d(texel) of all textures, packed like the textures
     std::vector<size_t> grad_off;           // per-texture offset in the arena (floats)
     DevBuf rec_key, rec_val, rec_scratch;   // deterministic-mode gradient records
@@ -922,6 +924,7 @@ int ohpt_zero_gradients(ohpt_context* c) {
         HIPCHK(c->grad_arena.ensure(std::max<size_t>(total, 64) * 4));
         c->grad_off = off;
     }
+    c->grad_total = total;
     launch_fill_u32(c->grad_arena.as<uint32_t>(), 0u, (uint32_t)std::max<size_t>(total, 64), c->stream);
     HIPCHK(hipGetLastError());
     if (changed) return upload_texdesc(c);
@@ -986,13 +989,21 @@ int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_
     HIPCHK(c->adj_hits.ensure((size_t)c->capacity * (c->st.max_bounces + 1) * 16));
     uint32_t* rec_key = nullptr;
     uint32_t* rec_val = nullptr;
+    uint32_t* rec_count = nullptr;
+    const uint64_t verts_per_path = std::max(1u, c->st.max_bounces);
     if (deterministic) {
         const size_t nrec = (size_t)npix * per_batch * recs_per_path;
+        const size_t nvert = (size_t)npix * per_batch * verts_per_path;
         HIPCHK(c->rec_key.ensure(nrec * 4));
         HIPCHK(c->rec_val.ensure(nrec * 4));
+        HIPCHK(c->rec_key2.ensure(nrec * 4));
+        HIPCHK(c->rec_val2.ensure(nrec * 4));
+        HIPCHK(c->rec_count.ensure(nvert * 4));
+        HIPCHK(c->rec_tiles.ensure(((size_t)record_tiles((uint32_t)nvert) + 1) * 4));
         HIPCHK(c->rec_scratch.ensure(radix_sort_scratch_bytes((uint32_t)nrec)));
         rec_key = c->rec_key.as<uint32_t>();
         rec_val = c->rec_val.as<uint32_t>();
+        rec_count = c->rec_count.as<uint32_t>();
     }
     const DevScene S = scene_view(c);
     PathState ps = path_state(c);
@@ -1016,8 +1027,8 @@ int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_
         f.flags = c->st.flags & ~(uint32_t)OHPT_FLAG_AOVS;
         f.firefly_clamp = c->st.firefly_clamp;
         f.ray_eps = c->st.ray_eps;
-        const uint32_t nrec = (uint32_t)(f.n_paths * recs_per_path);
-        if (deterministic) launch_fill_u32(rec_key, kNoRecord, nrec, c->stream);
+        const uint32_t nvert = (uint32_t)(f.n_paths * verts_per_path);
+        if (deterministic) launch_fill_u32(rec_count, 0u, nvert, c->stream);
         // lanes as in ohpt_render: each runs both passes over its own path range
         const int nl = f.n_paths >= (1u << 18) ? ohpt_context::kLanes : 1;
         const uint32_t part = ((f.n_paths + nl - 1) / nl + 63u) & ~63u;
@@ -1049,17 +1060,27 @@ int ohpt_adjoint(ohpt_context* c, const float* dl_dimage, int on_device, uint32_
             launch_fill_u32(pl.counters, 0u, kCounterWords, s);
             launch_generate(fl, pl, s);
             for (uint32_t b = 0; b <= f.max_bounces; ++b)
-                launch_shade_adjoint(S, fl, pl, b, vis, lrem, c->adj_dldi.as<float4>(), inv_spp, rec_key, rec_val, s);
+                launch_shade_adjoint(S, fl, pl, b, vis, lrem, c->adj_dldi.as<float4>(), inv_spp, rec_key, rec_val,
+                                     rec_count, s);
             if (nl > 1) {
                 HIPCHK(hipEventRecord(c->ev_lane[l], s));
                 HIPCHK(hipStreamWaitEvent(c->stream, c->ev_lane[l], 0));
             }
         }
         if (deterministic) {
-            // stable sort of the fixed-slot records by texel, then in-order segment sums
-            hipError_t e = radix_sort_pairs(rec_key, rec_val, nrec, 32, c->rec_scratch.p, c->stream);
+            // the written records in vertex order, a stable sort of them by texel (only the key
+            // bits the arena needs, whole 8-bit digits), then in-order segment sums
+            uint32_t nvalid = 0;
+            uint32_t* k2 = c->rec_key2.as<uint32_t>();
+            uint32_t* v2 = c->rec_val2.as<uint32_t>();
+            hipError_t e = compact_records(rec_count, nvert, rec_key, rec_val, c->rec_tiles.as<uint32_t>(), k2, v2,
+                                           &nvalid, c->stream);
+            if (e != hipSuccess) return fail_hip("compact_records", e);
+            uint32_t key_bits = 8;
+            while (key_bits < 32 && (uint64_t)c->grad_total > (1ull << key_bits)) key_bits += 8;
+            e = radix_sort_pairs(k2, v2, nvalid, key_bits, c->rec_scratch.p, c->stream);
             if (e != hipSuccess) return fail_hip("radix_sort_pairs", e);
-            launch_segment_reduce(rec_key, rec_val, nrec, c->grad_arena.as<float>(), c->stream);
+            launch_segment_reduce(k2, v2, nvalid, c->grad_arena.as<float>(), c->stream);
         }
         HIPCHK(hipGetLastError());
         sample += batch;

@@ -116,6 +116,7 @@ struct AdjointParams {
     float inv_spp;             // 1 / samples per pixel of the image being differentiated
     uint32_t* rec_key;         // deterministic mode: gradient records (null = atomics)
     uint32_t* rec_val;
+    uint32_t* rec_count;       // deterministic mode: records written per path vertex
 };
 
 // One shading step of the wavefront integrator. kAdjoint = false: forward
@@ -325,6 +326,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene S, FrameParams f
                     GradSink sink{adj.rec_key, adj.rec_val,
                                   (pid * f.max_bounces + bounce) * kGradRecordsPerVertex, 0u};
                     scatter_material_grad(S, m, s.mat, s.u, s.v, gA, gR, gM, sink);
+                    if (adj.rec_count) adj.rec_count[pid * f.max_bounces + bounce] = sink.n;
                 }
             }
         }
@@ -364,6 +366,58 @@ __global__ __launch_bounds__(256) void k_segment_reduce(const uint32_t* __restri
     float sum = 0.0f;
     for (uint32_t t = i; t < n && keys[t] == k; ++t) sum += __uint_as_float(vals[t]);
     arena[k] += sum;
+}
+
+// Deterministic compaction of the fixed-slot gradient records: vertex v owns
+// slots [v * kGradRecordsPerVertex, + count[v]); tiles of kRecTile vertices
+// count their records, an exclusive scan gives each tile its output offset,
+// and each thread copies its vertices' records in vertex order, so the
+// compacted stream (and every sum over it) does not depend on scheduling.
+constexpr uint32_t kRecItems = 16;
+constexpr uint32_t kRecTile = 256 * kRecItems;
+
+__global__ __launch_bounds__(256) void k_rec_tile_sum(const uint32_t* __restrict__ count, uint32_t nv,
+                                                      uint32_t* __restrict__ tile_sum) {
+    __shared__ uint32_t red[256];
+    const uint32_t v0 = blockIdx.x * kRecTile + threadIdx.x * kRecItems;
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < kRecItems; ++k)
+        if (v0 + k < nv) c += count[v0 + k];
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_rec_compact(const uint32_t* __restrict__ count, uint32_t nv,
+                                                     const uint32_t* __restrict__ tile_base,
+                                                     const uint32_t* __restrict__ key, const uint32_t* __restrict__ val,
+                                                     uint32_t* __restrict__ okey, uint32_t* __restrict__ oval) {
+    __shared__ uint32_t part[256];
+    const uint32_t v0 = blockIdx.x * kRecTile + threadIdx.x * kRecItems;
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < kRecItems; ++k)
+        if (v0 + k < nv) c += count[v0 + k];
+    part[threadIdx.x] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t x = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    uint32_t o = tile_base[blockIdx.x] + part[threadIdx.x] - c;
+    for (uint32_t k = 0; k < kRecItems && v0 + k < nv; ++k) {
+        const uint32_t n = count[v0 + k];
+        const size_t src = (size_t)(v0 + k) * kGradRecordsPerVertex;
+        for (uint32_t r = 0; r < n; ++r, ++o) {
+            okey[o] = key[src + r];
+            oval[o] = val[src + r];
+        }
+    }
 }
 
 // Remaining radiance for replay = the path's total radiance from the forward pass.
@@ -483,9 +537,25 @@ void launch_shade(const DevScene& S, const FrameParams& f, const PathState& ps, 
 
 void launch_shade_adjoint(const DevScene& S, const FrameParams& f, const PathState& ps, uint32_t bounce,
                           const uint32_t* vis, float4* lrem, const float4* dl_dimage, float inv_spp,
-                          uint32_t* rec_key, uint32_t* rec_val, hipStream_t s) {
-    const AdjointParams adj{vis, lrem, dl_dimage, inv_spp, rec_key, rec_val};
+                          uint32_t* rec_key, uint32_t* rec_val, uint32_t* rec_count, hipStream_t s) {
+    const AdjointParams adj{vis, lrem, dl_dimage, inv_spp, rec_key, rec_val, rec_count};
     k_shade<true><<<div_up(f.n_paths, kShadeBlock), kShadeBlock, 0, s>>>(S, f, ps, bounce, adj);
+}
+
+uint32_t record_tiles(uint32_t nv) { return div_up(nv, kRecTile); }
+
+hipError_t compact_records(const uint32_t* count, uint32_t nv, const uint32_t* key, const uint32_t* val,
+                           uint32_t* tiles, uint32_t* okey, uint32_t* oval, uint32_t* total_out, hipStream_t s) {
+    *total_out = 0;
+    if (nv == 0) return hipSuccess;
+    const uint32_t nt = record_tiles(nv);
+    k_rec_tile_sum<<<nt, 256, 0, s>>>(count, nv, tiles);
+    hipError_t e = exclusive_scan_u32(tiles, nt, tiles + nt, s);
+    if (e != hipSuccess) return e;
+    k_rec_compact<<<nt, 256, 0, s>>>(count, nv, tiles, key, val, okey, oval);
+    e = hipMemcpyAsync(total_out, tiles + nt, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
 }
 
 void launch_segment_reduce(const uint32_t* keys, const uint32_t* vals, uint32_t n, float* arena, hipStream_t s) {

@@ -535,6 +535,12 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t
     return hipGetLastError();
 }
 
+hipError_t exclusive_scan_u32(uint32_t* data, uint32_t n, uint32_t* total, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_scan_digit<<<1, 256, 0, s>>>(data, n, total);
+    return hipGetLastError();
+}
+
 size_t lbvh_scratch_bytes(uint32_t n) { return layout(n).total; }
 
 hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out, uint32_t* n4_out) {

@@ -35,6 +35,8 @@ hipError_t bvh4_collapse(const LbvhBuffers& b, float4* nodes4, hipStream_t s);
 
 // Standalone stable radix sort of 32-bit keys with 32-bit values (exposed for tests).
 size_t radix_sort_scratch_bytes(uint32_t n);
+// Exclusive scan of n u32 in place (one block; n up to a few 100k), total to *total (device).
+hipError_t exclusive_scan_u32(uint32_t* data, uint32_t n, uint32_t* total, hipStream_t s);
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, void* scratch,
                             hipStream_t s);
 
@@ -126,9 +128,15 @@ void launch_shade(const DevScene& S, const FrameParams& f, const PathState& ps, 
 // Path-replay adjoint of one shading step (same random numbers, recorded visibility).
 void launch_shade_adjoint(const DevScene& S, const FrameParams& f, const PathState& ps, uint32_t bounce,
                           const uint32_t* vis, float4* lrem, const float4* dl_dimage, float inv_spp,
-                          uint32_t* rec_key, uint32_t* rec_val, hipStream_t s);
+                          uint32_t* rec_key, uint32_t* rec_val, uint32_t* rec_count, hipStream_t s);
 // Deterministic mode: sums runs of equal keys (sorted records) into arena[key].
 void launch_segment_reduce(const uint32_t* keys, const uint32_t* vals, uint32_t n, float* arena, hipStream_t s);
+// Deterministic mode: compacts the records of nv path vertices (count[v] of them in
+// vertex v's fixed slots) into okey/oval in vertex order; *total_out (host) = records.
+// tiles: record_tiles(nv) + 1 words. Synchronous.
+uint32_t record_tiles(uint32_t nv);
+hipError_t compact_records(const uint32_t* count, uint32_t nv, const uint32_t* key, const uint32_t* val,
+                           uint32_t* tiles, uint32_t* okey, uint32_t* oval, uint32_t* total_out, hipStream_t s);
 // lrem[pid] = total radiance of path pid from the forward pass
 // lrem[pid] = L + light-NEE + env-NEE radiance for pid in [pid_base, pid_base + n)
 void launch_adjoint_init(const PathState& ps, uint32_t pid_base, uint32_t n, float4* lrem, hipStream_t s);
