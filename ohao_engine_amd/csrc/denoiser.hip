@@ -9,11 +9,14 @@
 // accumulation; bias + ReLU are fused into the forward epilogue.
 //
 // Kernel shapes (wave64, 256-thread blocks = 4 waves):
-//   k_conv3x3<CC>  one block = 16x8 output pixels x 32 output channels. Per
-//                  input-channel chunk of CC channels the 18x10 input halo and
-//                  the chunk's packed weights are staged in LDS; each wave
-//                  owns two 16-pixel rows x 32 channels (2x2 MFMA tiles) and
-//                  walks K = 9 taps x CC channels in steps of 32.
+//   k_conv3x3<CC,MX> one block = (16*MX)x8 output pixels x 32 output channels.
+//                  Per input-channel chunk of CC channels the (16*MX+2)x10
+//                  input halo and the chunk's packed weights are staged in
+//                  LDS; each wave owns two rows of 16*MX pixels x 32 channels
+//                  (2*MX x 2 MFMA tiles, each weight fragment used 2*MX times)
+//                  and walks K = 9 taps x CC channels in steps of 32; the bf16
+//                  result is transposed through LDS so each output pixel's 32
+//                  channels are written as four 16-byte stores.
 //   k_wgrad<CC>    one block = (pixel-tile range, 32 output channels, CC
 //                  input channels); K = pixels, 32 per MFMA step; per-block
 //                  partial sums go to a buffer that a fixed-order reduction
@@ -77,27 +80,33 @@ struct ConvArgs {
     int B, H, W;
 };
 
-template <int CC>
+template <int CC, int MX>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     constexpr int KS = (9 * CC + 31) / 32;   // K steps of 32 per input chunk
     constexpr int XS = CC + 8;               // LDS halo pixel stride (bf16 elements)
     constexpr int WS = 40;                   // LDS weight row stride
     constexpr int VPP = CC / 8;              // 16-byte vectors per halo pixel
-    __shared__ __align__(16) uint16_t xs[180 * XS];
+    constexpr int TX = 16 * MX;              // output pixels per block row
+    constexpr int HX = TX + 2;               // halo row length
+    constexpr int HP = HX * 10;              // halo pixels
+    __shared__ __align__(16) uint16_t xs[HP * XS];
     __shared__ __align__(16) uint16_t ws[KS * 32 * WS];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, r16 = lane & 15;
     const int ncout = a.np >> 5;
     const int b = blockIdx.z / ncout, nc = blockIdx.z - b * ncout;
-    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 8;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * 8;
     const int nchunk = (a.c0 + a.c1) / CC;
 
-    f32x4 acc[2][2];
+    // this wave: rows 2*wave, 2*wave+1; MX 16-pixel tiles per row; 2 tiles of 16 output channels
+    f32x4 acc[2][MX][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int m = 0; m < MX; ++m)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     for (int c = 0; c < nchunk; ++c) {
         const int cb = c * CC;
@@ -106,9 +115,9 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
         if (cb < a.c0) { src = a.in0; cs = a.c0; co = cb; }
         else { src = a.in1; cs = a.c1; co = cb - a.c0; }
         __syncthreads();
-        for (int i = tid; i < 180 * VPP; i += 256) {
+        for (int i = tid; i < HP * VPP; i += 256) {
             const int p = i / VPP, v = i - p * VPP;
-            const int hy = p / 18, hx = p - hy * 18;
+            const int hy = p / HX, hx = p - hy * HX;
             const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
             uint4 val = make_uint4(0u, 0u, 0u, 0u);
             if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
@@ -126,49 +135,89 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
         for (int s = 0; s < KS; ++s) {
             const int kf = s * 32 + g * 8;
             const int tap = kf / CC, ci0 = kf - tap * CC;
-            bf16x8 af[2], bfr[2];
-#pragma unroll
-            for (int mi = 0; mi < 2; ++mi) {
-                const int row = wave * 2 + mi;
-                uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                if (tap < 9) {
-                    const int dy = tap / 3, dx = tap - dy * 3;
-                    v = *reinterpret_cast<const uint4*>(&xs[((row + dy) * 18 + r16 + dx) * XS + ci0]);
-                }
-                af[mi] = as_bf16x8(v);
-            }
+            const int dy = tap / 3, dx = tap - dy * 3;
+            bf16x8 bfr[2];
 #pragma unroll
             for (int ni = 0; ni < 2; ++ni)
                 bfr[ni] = as_bf16x8(*reinterpret_cast<const uint4*>(&ws[(s * 32 + ni * 16 + r16) * WS + g * 8]));
 #pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
+            for (int mi = 0; mi < 2; ++mi) {
+                const int row = wave * 2 + mi;
 #pragma unroll
-                for (int ni = 0; ni < 2; ++ni)
-                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mi], bfr[ni], acc[mi][ni], 0, 0, 0);
+                for (int m = 0; m < MX; ++m) {
+                    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                    if (tap < 9) v = *reinterpret_cast<const uint4*>(&xs[((row + dy) * HX + m * 16 + r16 + dx) * XS + ci0]);
+                    const bf16x8 af = as_bf16x8(v);
+#pragma unroll
+                    for (int ni = 0; ni < 2; ++ni)
+                        acc[mi][m][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[ni], acc[mi][m][ni], 0, 0, 0);
+                }
+            }
         }
     }
 
-    // D layout of a 16x16 tile: lane (g, r16), register i -> pixel x = 4g + i, channel r16
+    // D layout of a 16x16 tile: lane (g, r16), register i -> pixel x = 4g + i, channel r16.
+    // FP32 logits are written directly; bf16 activations go through LDS below.
+    if (a.outf) {
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi) {
-        const int y = y0 + wave * 2 + mi;
-        if (y >= a.H) continue;
+        for (int mi = 0; mi < 2; ++mi) {
+            const int y = y0 + wave * 2 + mi;
+            if (y >= a.H) continue;
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni) {
-            const int n = nc * 32 + ni * 16 + r16;
-            const float bias = (a.bias && n < a.nb) ? a.bias[n] : 0.0f;
+            for (int m = 0; m < MX; ++m) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int x = x0 + g * 4 + i;
-                if (x >= a.W) continue;
-                float v = acc[mi][ni][i] + bias;
-                if (a.relu) v = fmaxf(v, 0.0f);
-                const size_t pix = (size_t)(b * a.H + y) * a.W + x;
-                if (a.outf) a.outf[pix * a.np + n] = v;
-                else if (n < a.oc0) a.out0[pix * a.oc0 + n] = f2bf(v);
-                else a.out1[pix * a.oc1 + (n - a.oc0)] = f2bf(v);
+                for (int ni = 0; ni < 2; ++ni) {
+                    const int n = nc * 32 + ni * 16 + r16;
+                    const float bias = (a.bias && n < a.nb) ? a.bias[n] : 0.0f;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int x = x0 + m * 16 + g * 4 + i;
+                        if (x >= a.W) continue;
+                        float v = acc[mi][m][ni][i] + bias;
+                        if (a.relu) v = fmaxf(v, 0.0f);
+                        const size_t pix = (size_t)(b * a.H + y) * a.W + x;
+                        if (a.outf) a.outf[pix * a.np + n] = v;
+                        else if (n < a.oc0) a.out0[pix * a.oc0 + n] = f2bf(v);
+                        else a.out1[pix * a.oc1 + (n - a.oc0)] = f2bf(v);
+                    }
+                }
             }
         }
+        return;
+    }
+    // bf16 outputs: transpose the tile through LDS ([pixel][32 channels], padded) so each
+    // pixel's 32 channels leave as four 16-byte stores
+    constexpr int OS = 40;
+    static_assert(8 * TX * OS <= KS * 32 * WS, "output staging fits the weight buffer");
+    uint16_t* os = ws;
+    __syncthreads();
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi) {
+#pragma unroll
+        for (int m = 0; m < MX; ++m) {
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                const int nl = ni * 16 + r16, n = nc * 32 + nl;
+                const float bias = (a.bias && n < a.nb) ? a.bias[n] : 0.0f;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    float v = acc[mi][m][ni][i] + bias;
+                    if (a.relu) v = fmaxf(v, 0.0f);
+                    os[((wave * 2 + mi) * TX + m * 16 + g * 4 + i) * OS + nl] = f2bf(v);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < 8 * TX * 4; i += 256) {
+        const int pl = i >> 2, v = i & 3;
+        const int y = y0 + pl / TX, x = x0 + pl % TX;
+        if (y >= a.H || x >= a.W) continue;
+        const uint4 val = *reinterpret_cast<const uint4*>(&os[pl * OS + v * 8]);
+        const int n = nc * 32 + v * 8;
+        const size_t pix = (size_t)(b * a.H + y) * a.W + x;
+        if (n < a.oc0) *reinterpret_cast<uint4*>(&a.out0[pix * a.oc0 + n]) = val;
+        else *reinterpret_cast<uint4*>(&a.out1[pix * a.oc1 + (n - a.oc0)]) = val;
     }
 }
 
@@ -753,8 +802,12 @@ hipError_t ensure_capacity(ohpt_denoiser* d, int B, int Hp, int Wp) {
     return e;
 }
 
+// 16-pixel tiles per block row: 1 keeps LDS at 37 KB (4 blocks per CU); wider
+// tiles reuse each weight fragment more but fit fewer blocks and measured slower.
+constexpr int kConvMX = 1;
+
 void launch_conv(ohpt_denoiser* d, const Layer& L, const ConvArgs& a, int cc) {
-    dim3 grid(cdiv(a.W, 16), cdiv(a.H, 8), (uint32_t)(a.B * (a.np / 32)));
+    dim3 grid(cdiv(a.W, 16 * kConvMX), cdiv(a.H, 8), (uint32_t)(a.B * (a.np / 32)));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (d->profile) {
         if ((int)d->conv_ev.size() < d->conv_ev_used + 2) {
@@ -769,8 +822,8 @@ void launch_conv(ohpt_denoiser* d, const Layer& L, const ConvArgs& a, int cc) {
         d->conv_ev_used += 2;
         (void)hipEventRecord(e0, d->stream);
     }
-    if (cc == 16) k_conv3x3<16><<<grid, 256, 0, d->stream>>>(a);
-    else k_conv3x3<32><<<grid, 256, 0, d->stream>>>(a);
+    if (cc == 16) k_conv3x3<16, kConvMX><<<grid, 256, 0, d->stream>>>(a);
+    else k_conv3x3<32, kConvMX><<<grid, 256, 0, d->stream>>>(a);
     if (d->profile) (void)hipEventRecord(e1, d->stream);
     (void)L;
 }
