@@ -43,12 +43,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------- bf16 helpers
-__device__ inline uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);   // quiet NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                                             // round to nearest even
-    return (uint16_t)(u >> 16);
-}
+// Round to nearest even (gfx950 v_cvt_pk_bf16_f32); a NaN stays a quiet NaN.
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 inline uint16_t f2bf_host(float f) {
@@ -115,14 +111,23 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
         if (cb < a.c0) { src = a.in0; cs = a.c0; co = cb; }
         else { src = a.in1; cs = a.c1; co = cb - a.c0; }
         __syncthreads();
-        for (int i = tid; i < HP * VPP; i += 256) {
-            const int p = i / VPP, v = i - p * VPP;
-            const int hy = p / HX, hx = p - hy * HX;
-            const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
-            uint4 val = make_uint4(0u, 0u, 0u, 0u);
-            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-                val = *reinterpret_cast<const uint4*>(src + ((size_t)(b * a.H + gy) * a.W + gx) * cs + co + v * 8);
-            *reinterpret_cast<uint4*>(&xs[p * XS + v * 8]) = val;
+        {
+            // thread -> (halo pixel p, 16-byte vector v); p advances by 256 / VPP per step
+            constexpr int PSTEP = 256 / VPP, DY = PSTEP / HX, DX = PSTEP - DY * HX;
+            const int v = tid % VPP;
+            int p = tid / VPP;
+            int hy = p / HX, hx = p - hy * HX;
+            const uint16_t* img = src + (size_t)b * a.H * a.W * cs + co + v * 8;
+            for (; p < HP; p += PSTEP) {
+                const int gy = y0 - 1 + hy, gx = x0 - 1 + hx;
+                uint4 val = make_uint4(0u, 0u, 0u, 0u);
+                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                    val = *reinterpret_cast<const uint4*>(img + ((size_t)gy * a.W + gx) * cs);
+                *reinterpret_cast<uint4*>(&xs[p * XS + v * 8]) = val;
+                hy += DY;
+                hx += DX;
+                if (hx >= HX) { hx -= HX; ++hy; }
+            }
         }
         const uint16_t* wsrc = a.wp + ((size_t)c * KS * a.np + (size_t)nc * 32) * 32;
         for (int i = tid; i < KS * 128; i += 256) {
@@ -131,7 +136,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
                 *reinterpret_cast<const uint4*>(wsrc + ((size_t)s * a.np + n) * 32 + v * 8);
         }
         __syncthreads();
-#pragma unroll 3
+#pragma unroll
         for (int s = 0; s < KS; ++s) {
             const int kf = s * 32 + g * 8;
             const int tap = kf / CC, ci0 = kf - tap * CC;
