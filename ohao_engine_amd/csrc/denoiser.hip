@@ -79,8 +79,10 @@ struct ConvArgs {
 template <int CC, int MX>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     constexpr int KS = (9 * CC + 31) / 32;   // K steps of 32 per input chunk
-    constexpr int XS = CC + 8;               // LDS halo pixel stride (bf16 elements)
-    constexpr int WS = 40;                   // LDS weight row stride
+    // LDS strides (bf16 elements), measured: a halo pixel padded by 8 and unpadded
+    // weight rows beat the other 16-byte-aligned choices (0/16/24 and 40/48/56)
+    constexpr int XS = CC + 8;               // halo pixel stride
+    constexpr int WS = 32;                   // weight row stride
     constexpr int VPP = CC / 8;              // 16-byte vectors per halo pixel
     constexpr int TX = 16 * MX;              // output pixels per block row
     constexpr int HX = TX + 2;               // halo row length
