@@ -162,6 +162,14 @@ int ohpt_scene_add_mesh(ohpt_context* ctx,
                         const uint32_t* mat_ids, uint32_t material_base,
                         const float* world /* 16 floats or NULL */);
 int ohpt_scene_commit(ohpt_context* ctx);
+/* Transform change without a rebuild (the TLAS instance-transform update of
+ * rt_acceleration_structure.cpp:419-535): mesh = index in add order, world as
+ * in ohpt_scene_add_mesh (NULL = identity). ohpt_scene_refit then refits the
+ * committed tree: same topology and leaf order, every box recomputed from the
+ * new triangles, the 4-wide nodes re-collapsed. Large motions degrade the
+ * tree's quality, not its correctness; commit again to rebuild. */
+int ohpt_scene_set_mesh_transform(ohpt_context* ctx, uint32_t mesh, const float* world);
+int ohpt_scene_refit(ohpt_context* ctx);
 int ohpt_scene_triangle_count(ohpt_context* ctx, uint64_t* out);
 /* Maximum leaf depth of the committed LBVH (root = 1). */
 int ohpt_scene_tree_depth(ohpt_context* ctx, uint32_t* out);

@@ -57,6 +57,17 @@ struct ohpt_context {
 
     // scene staging (host, caller's global triangle order)
     bool in_scene = false;
+    // one record per added mesh: its object-space data and triangle range, so a
+    // new world matrix can be applied later (ohpt_scene_set_mesh_transform)
+    struct MeshRec {
+        uint32_t base = 0, nt = 0;
+        std::vector<float> pos, nrm;    // object space, per vertex
+        std::vector<uint32_t> idx;
+        double world[16];
+        bool has_world = false;
+    };
+    std::vector<MeshRec> meshes;
+    bool refit_pending = false;   // a mesh transform changed since the last commit/refit
     std::vector<float> h_pos, h_nrm, h_uv;
     std::vector<uint32_t> h_mat;
     bool any_nrm = false, any_uv = false;
@@ -400,9 +411,59 @@ int ohpt_resize(ohpt_context* c, uint32_t width, uint32_t height) {
 }
 
 // ---------------------------------------------------------------- scene
+extern "C++" {
+namespace {
+// World-space triangles (and normals) of one mesh record into h_pos / h_nrm.
+void emit_mesh(ohpt_context* c, const ohpt_context::MeshRec& m) {
+    const double* M = m.world;
+    // normal matrix = inverse-transpose of the upper 3x3 (pt_closesthit.rchit:66)
+    const double a = M[0], b = M[4], cc = M[8], d = M[1], e = M[5], f = M[9], g = M[2], h = M[6], i = M[10];
+    const double det = a * (e * i - f * h) - b * (d * i - f * g) + cc * (d * h - e * g);
+    double N[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major N[r*3+col] applied as n' = N n
+    if (std::fabs(det) > 0) {
+        N[0] = (e * i - f * h) / det; N[1] = -(d * i - f * g) / det; N[2] = (d * h - e * g) / det;
+        N[3] = -(b * i - cc * h) / det; N[4] = (a * i - cc * g) / det; N[5] = -(a * h - b * g) / det;
+        N[6] = (b * f - cc * e) / det; N[7] = -(a * f - cc * d) / det; N[8] = (a * e - b * d) / det;
+    }
+    const bool has_n = !m.nrm.empty();
+    for (uint32_t t = 0; t < m.nt; ++t) {
+        float* P = &c->h_pos[((size_t)m.base + t) * 9];
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t vi = m.idx[(size_t)t * 3 + k];
+            const double x = m.pos[(size_t)vi * 3], y = m.pos[(size_t)vi * 3 + 1], z = m.pos[(size_t)vi * 3 + 2];
+            if (m.has_world) {
+                P[k * 3 + 0] = (float)(M[0] * x + M[4] * y + M[8] * z + M[12]);
+                P[k * 3 + 1] = (float)(M[1] * x + M[5] * y + M[9] * z + M[13]);
+                P[k * 3 + 2] = (float)(M[2] * x + M[6] * y + M[10] * z + M[14]);
+            } else {
+                P[k * 3 + 0] = (float)x; P[k * 3 + 1] = (float)y; P[k * 3 + 2] = (float)z;
+            }
+            if (c->any_nrm && has_n) {
+                float* Nn = &c->h_nrm[((size_t)m.base + t) * 9 + k * 3];
+                const double nx = m.nrm[(size_t)vi * 3], ny = m.nrm[(size_t)vi * 3 + 1], nz = m.nrm[(size_t)vi * 3 + 2];
+                double o0 = nx, o1 = ny, o2 = nz;
+                if (m.has_world) {
+                    o0 = N[0] * nx + N[1] * ny + N[2] * nz;
+                    o1 = N[3] * nx + N[4] * ny + N[5] * nz;
+                    o2 = N[6] * nx + N[7] * ny + N[8] * nz;
+                }
+                Nn[0] = (float)o0; Nn[1] = (float)o1; Nn[2] = (float)o2;
+            }
+        }
+        if (c->any_nrm && !has_n) {
+            const float e1[3] = {P[3] - P[0], P[4] - P[1], P[5] - P[2]}, e2[3] = {P[6] - P[0], P[7] - P[1], P[8] - P[2]};
+            const float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            for (int k = 0; k < 3; ++k) for (int q = 0; q < 3; ++q) c->h_nrm[((size_t)m.base + t) * 9 + k * 3 + q] = n[q];
+        }
+    }
+}
+}  // namespace
+}  // extern "C++"
+
 int ohpt_scene_begin(ohpt_context* c) {
     if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_begin: null context");
     c->in_scene = true;
+    c->meshes.clear();
     c->h_pos.clear(); c->h_nrm.clear(); c->h_uv.clear(); c->h_mat.clear();
     c->any_nrm = c->any_uv = false;
     c->n_tris = 0;
@@ -417,29 +478,21 @@ int ohpt_scene_add_mesh(ohpt_context* c, const float* positions, const float* no
     if ((uint64_t)c->n_tris + nt >= 0x7FFFFFFFull) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_add_mesh: too many triangles");
     for (uint64_t i = 0; i < (uint64_t)nt * 3; ++i)
         if (indices[i] >= nv) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_add_mesh: index out of range");
-    double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-    if (world) for (int k = 0; k < 16; ++k) M[k] = world[k];
-    // normal matrix = inverse-transpose of the upper 3x3 (pt_closesthit.rchit:66)
-    const double a = M[0], b = M[4], cc = M[8], d = M[1], e = M[5], f = M[9], g = M[2], h = M[6], i = M[10];
-    const double det = a * (e * i - f * h) - b * (d * i - f * g) + cc * (d * h - e * g);
-    double N[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major N[r*3+col] applied as n' = N n
-    if (std::fabs(det) > 0) {
-        // inverse transpose rows
-        N[0] = (e * i - f * h) / det; N[1] = -(d * i - f * g) / det; N[2] = (d * h - e * g) / det;
-        N[3] = -(b * i - cc * h) / det; N[4] = (a * i - cc * g) / det; N[5] = -(a * h - b * g) / det;
-        N[6] = (b * f - cc * e) / det; N[7] = -(a * f - cc * d) / det; N[8] = (a * e - b * d) / det;
-    }
     const bool had_nrm = c->any_nrm, had_uv = c->any_uv;
     const size_t base = c->n_tris;
+    ohpt_context::MeshRec rec;
+    rec.base = (uint32_t)base;
+    rec.nt = nt;
+    rec.pos.assign(positions, positions + (size_t)nv * 3);
+    if (normals) rec.nrm.assign(normals, normals + (size_t)nv * 3);
+    rec.idx.assign(indices, indices + (size_t)nt * 3);
+    rec.has_world = world != nullptr;
+    for (int k = 0; k < 16; ++k) rec.world[k] = world ? (double)world[k] : (k % 5 == 0 ? 1.0 : 0.0);
     if (normals && !had_nrm && base > 0) {
         // earlier meshes had no normals: give them their geometric normals
         c->h_nrm.assign(base * 9, 0.0f);
-        for (size_t t = 0; t < base; ++t) {
-            const float* p = &c->h_pos[t * 9];
-            const float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]}, e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
-            const float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-            for (int k = 0; k < 3; ++k) for (int q = 0; q < 3; ++q) c->h_nrm[t * 9 + k * 3 + q] = n[q];
-        }
+        c->any_nrm = true;
+        for (const auto& m : c->meshes) emit_mesh(c, m);
     }
     if (normals) c->any_nrm = true;
     if (uvs && !had_uv) { c->h_uv.assign(base * 6, 0.0f); c->any_uv = true; }
@@ -448,45 +501,18 @@ int ohpt_scene_add_mesh(ohpt_context* c, const float* positions, const float* no
     if (c->any_uv) c->h_uv.resize((base + nt) * 6, 0.0f);
     c->h_mat.resize(base + nt);
     for (uint32_t t = 0; t < nt; ++t) {
-        float* P = &c->h_pos[(base + t) * 9];
         for (int k = 0; k < 3; ++k) {
             const uint32_t vi = indices[(size_t)t * 3 + k];
-            const double x = positions[(size_t)vi * 3], y = positions[(size_t)vi * 3 + 1], z = positions[(size_t)vi * 3 + 2];
-            if (world) {
-                P[k * 3 + 0] = (float)(M[0] * x + M[4] * y + M[8] * z + M[12]);
-                P[k * 3 + 1] = (float)(M[1] * x + M[5] * y + M[9] * z + M[13]);
-                P[k * 3 + 2] = (float)(M[2] * x + M[6] * y + M[10] * z + M[14]);
-            } else {
-                P[k * 3 + 0] = (float)x; P[k * 3 + 1] = (float)y; P[k * 3 + 2] = (float)z;
-            }
-            if (c->any_nrm) {
-                float* Nn = &c->h_nrm[(base + t) * 9 + k * 3];
-                if (normals) {
-                    const double nx = normals[(size_t)vi * 3], ny = normals[(size_t)vi * 3 + 1], nz = normals[(size_t)vi * 3 + 2];
-                    double o0 = nx, o1 = ny, o2 = nz;
-                    if (world) {
-                        o0 = N[0] * nx + N[1] * ny + N[2] * nz;
-                        o1 = N[3] * nx + N[4] * ny + N[5] * nz;
-                        o2 = N[6] * nx + N[7] * ny + N[8] * nz;
-                    }
-                    Nn[0] = (float)o0; Nn[1] = (float)o1; Nn[2] = (float)o2;
-                } else {
-                    Nn[0] = Nn[1] = Nn[2] = 0.0f;  // filled with the geometric normal below
-                }
-            }
             if (c->any_uv) {
                 float* U = &c->h_uv[(base + t) * 6 + k * 2];
                 U[0] = uvs ? uvs[(size_t)vi * 2] : 0.0f;
                 U[1] = uvs ? uvs[(size_t)vi * 2 + 1] : 0.0f;
             }
         }
-        if (c->any_nrm && !normals) {
-            const float e1[3] = {P[3] - P[0], P[4] - P[1], P[5] - P[2]}, e2[3] = {P[6] - P[0], P[7] - P[1], P[8] - P[2]};
-            const float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-            for (int k = 0; k < 3; ++k) for (int q = 0; q < 3; ++q) c->h_nrm[(base + t) * 9 + k * 3 + q] = n[q];
-        }
         c->h_mat[base + t] = material_base + (mat_ids ? mat_ids[t] : 0u);
     }
+    emit_mesh(c, rec);
+    c->meshes.push_back(std::move(rec));
     c->n_tris += nt;
     return OHPT_OK;
 }
@@ -553,6 +579,60 @@ int ohpt_scene_commit(ohpt_context* c) {
     c->n_nodes4 = n4;
     c->committed = true;
     c->in_scene = false;
+    return build_emitters(c);
+}
+
+int ohpt_scene_set_mesh_transform(ohpt_context* c, uint32_t mesh, const float* world) {
+    if (!c) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_set_mesh_transform: null context");
+    if (mesh >= c->meshes.size()) return fail(OHPT_ERR_INVALID_ARG, "ohpt_scene_set_mesh_transform: no such mesh");
+    ohpt_context::MeshRec& m = c->meshes[mesh];
+    m.has_world = world != nullptr;
+    for (int k = 0; k < 16; ++k) m.world[k] = world ? (double)world[k] : (k % 5 == 0 ? 1.0 : 0.0);
+    emit_mesh(c, m);
+    c->refit_pending = true;
+    return OHPT_OK;
+}
+
+int ohpt_scene_refit(ohpt_context* c) {
+    if (!c || !c->committed) return fail(OHPT_ERR_NO_SCENE, "ohpt_scene_refit: no committed scene");
+    HIPCHK(hipSetDevice(c->device));
+    const uint32_t n = c->n_tris;
+    c->refit_pending = false;
+    if (n == 0) return OHPT_OK;
+    DevBuf pos, nrm, uv, mat, scratch;
+    HIPCHK(pos.ensure((size_t)n * 36));
+    HIPCHK(mat.ensure((size_t)n * 4));
+    HIPCHK(hipMemcpyAsync(pos.p, c->h_pos.data(), (size_t)n * 36, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(mat.p, c->h_mat.data(), (size_t)n * 4, hipMemcpyHostToDevice, c->stream));
+    if (c->any_nrm) {
+        HIPCHK(nrm.ensure((size_t)n * 36));
+        HIPCHK(hipMemcpyAsync(nrm.p, c->h_nrm.data(), (size_t)n * 36, hipMemcpyHostToDevice, c->stream));
+    }
+    if (c->any_uv) {
+        HIPCHK(uv.ensure((size_t)n * 24));
+        HIPCHK(hipMemcpyAsync(uv.p, c->h_uv.data(), (size_t)n * 24, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(scratch.ensure(lbvh_scratch_bytes(n)));
+    LbvhBuffers b{};
+    b.tri_pos = pos.as<float>();
+    b.tri_nrm = c->any_nrm ? nrm.as<float>() : nullptr;
+    b.tri_uv = c->any_uv ? uv.as<float>() : nullptr;
+    b.tri_mat = mat.as<uint32_t>();
+    b.n = n;
+    b.nodes = c->d_nodes.as<float4>();
+    b.tris = c->d_tris.as<float4>();
+    b.attrs = c->d_attrs.as<float4>();
+    b.prim = c->d_prim.as<uint32_t>();
+    b.slot_of = c->d_slot_of.as<uint32_t>();
+    b.scratch = scratch.p;
+    HIPCHK(hipEventRecord(c->ev_start, c->stream));
+    hipError_t e = lbvh_refit(b, c->stream);
+    if (e != hipSuccess) return fail_hip("lbvh_refit", e);
+    e = bvh4_collapse(b, c->d_nodes4.as<float4>(), c->stream);
+    if (e != hipSuccess) return fail_hip("bvh4_collapse", e);
+    HIPCHK(hipEventRecord(c->ev_stop, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev_stop));
+    c->stats.ms_build = elapsed(c->ev_start, c->ev_stop);
     return build_emitters(c);
 }
 

@@ -543,6 +543,74 @@ hipError_t exclusive_scan_u32(uint32_t* data, uint32_t n, uint32_t* total, hipSt
 
 size_t lbvh_scratch_bytes(uint32_t n) { return layout(n).total; }
 
+__global__ __launch_bounds__(256) void k_parent_from_nodes(const uint32_t* __restrict__ nodes_u, uint32_t ni,
+                                                          uint32_t* __restrict__ node_parent) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < ni) node_parent[i] = nodes_u[(size_t)i * 16 + 14];
+}
+
+// Depth buckets, level-by-level refit and the 4-wide numbering, shared by the
+// build and the refit (node_parent filled, topology in b.nodes).
+hipError_t refit_levels(const LbvhBuffers& b, const ScratchLayout& L, hipStream_t s, uint32_t* depth_out,
+                        uint32_t* n4_out) {
+    const uint32_t n = b.n;
+    char* sc = reinterpret_cast<char*>(b.scratch);
+    uint32_t* node_parent = reinterpret_cast<uint32_t*>(sc + L.node_parent);
+    uint32_t* node_depth = reinterpret_cast<uint32_t*>(sc + L.node_depth);
+    uint32_t* level_count = reinterpret_cast<uint32_t*>(sc + L.levels);
+    uint32_t* level_cursor = level_count + kMaxLevels;
+    uint32_t* level_nodes = reinterpret_cast<uint32_t*>(sc + L.level_nodes);
+    const uint32_t ni = n > 1 ? n - 1 : 1;
+    k_fill<<<1, 256, 0, s>>>(level_count, 0u, 2 * kMaxLevels);
+    k_node_depth<<<div_up(ni, 256), 256, 0, s>>>(node_parent, ni, node_depth, level_count);
+    uint32_t counts[kMaxLevels];
+    hipError_t e = hipMemcpyAsync(counts, level_count, sizeof(counts), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    uint32_t offs[kMaxLevels], maxd = 0, acc = 0, n4 = 0;
+    for (uint32_t d = 0; d < kMaxLevels; ++d) {
+        offs[d] = acc;
+        acc += counts[d];
+        if (counts[d]) maxd = d;
+        if ((d & 1u) == 0) n4 += counts[d];
+    }
+    e = hipMemcpyAsync(level_cursor, offs, sizeof(offs), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    k_level_scatter<<<div_up(ni, 256), 256, 0, s>>>(node_depth, ni, level_cursor, level_nodes);
+    for (int d = (int)maxd; d >= 0; --d) {
+        if (counts[d] == 0) continue;
+        k_refit_level<<<div_up(counts[d], 256), 256, 0, s>>>(b.tris, reinterpret_cast<float*>(b.nodes), node_parent,
+                                                              level_nodes + offs[d], counts[d]);
+    }
+    // 4-wide numbering: even-depth flags -> per-tile counts -> scan -> ranks
+    const uint32_t nt4 = div_up(ni, kCollapseTile);
+    uint32_t* tiles4 = reinterpret_cast<uint32_t*>(sc + L.tiles4);
+    uint32_t* map4 = reinterpret_cast<uint32_t*>(sc + L.map4);
+    k_even_count<<<nt4, 256, 0, s>>>(node_depth, ni, tiles4);
+    k_scan_digit<<<1, 256, 0, s>>>(tiles4, nt4, tiles4 + nt4);
+    k_even_map<<<nt4, 256, 0, s>>>(node_depth, ni, tiles4, map4);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // deepest leaf = two levels below the deepest internal node (root at 0, leaves counted)
+    if (depth_out) *depth_out = maxd + 2;
+    if (n4_out) *n4_out = n4;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t lbvh_refit(const LbvhBuffers& b, hipStream_t s) {
+    const uint32_t n = b.n;
+    if (n == 0) return hipSuccess;
+    const ScratchLayout L = layout(n);
+    char* sc = reinterpret_cast<char*>(b.scratch);
+    uint32_t* node_parent = reinterpret_cast<uint32_t*>(sc + L.node_parent);
+    const uint32_t ni = n > 1 ? n - 1 : 1;
+    // new vertex data into the committed leaf order (b.prim: slot -> triangle id)
+    k_gather<<<div_up(n, 256), 256, 0, s>>>(b.tri_pos, b.tri_nrm, b.tri_uv, b.tri_mat, b.prim, n, b.tris, b.attrs,
+                                             reinterpret_cast<uint32_t*>(sc + L.vals0), b.slot_of);
+    k_parent_from_nodes<<<div_up(ni, 256), 256, 0, s>>>(reinterpret_cast<const uint32_t*>(b.nodes), ni, node_parent);
+    return refit_levels(b, L, s, nullptr, nullptr);
+}
+
 hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out, uint32_t* n4_out) {
     const uint32_t n = b.n;
     if (n == 0) return hipSuccess;
@@ -586,40 +654,7 @@ hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out, 
         k_karras<<<div_up(n - 1, 256), 256, 0, s>>>(ka, va, n, nodes_u, node_parent, leaf_parent);
     }
     // bucket internal nodes by depth, then refit level by level, deepest first
-    k_fill<<<1, 256, 0, s>>>(level_count, 0u, 2 * kMaxLevels);
-    k_node_depth<<<div_up(ni, 256), 256, 0, s>>>(node_parent, ni, node_depth, level_count);
-    uint32_t counts[kMaxLevels];
-    hipError_t e = hipMemcpyAsync(counts, level_count, sizeof(counts), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
-    uint32_t offs[kMaxLevels], maxd = 0, acc = 0, n4 = 0;
-    for (uint32_t d = 0; d < kMaxLevels; ++d) {
-        offs[d] = acc;
-        acc += counts[d];
-        if (counts[d]) maxd = d;
-        if ((d & 1u) == 0) n4 += counts[d];
-    }
-    e = hipMemcpyAsync(level_cursor, offs, sizeof(offs), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-    k_level_scatter<<<div_up(ni, 256), 256, 0, s>>>(node_depth, ni, level_cursor, level_nodes);
-    for (int d = (int)maxd; d >= 0; --d) {
-        if (counts[d] == 0) continue;
-        k_refit_level<<<div_up(counts[d], 256), 256, 0, s>>>(b.tris, reinterpret_cast<float*>(b.nodes), node_parent,
-                                                              level_nodes + offs[d], counts[d]);
-    }
-    // 4-wide numbering: even-depth flags -> per-tile counts -> scan -> ranks
-    const uint32_t nt4 = div_up(ni, kCollapseTile);
-    uint32_t* tiles4 = reinterpret_cast<uint32_t*>(sc + L.tiles4);
-    uint32_t* map4 = reinterpret_cast<uint32_t*>(sc + L.map4);
-    k_even_count<<<nt4, 256, 0, s>>>(node_depth, ni, tiles4);
-    k_scan_digit<<<1, 256, 0, s>>>(tiles4, nt4, tiles4 + nt4);
-    k_even_map<<<nt4, 256, 0, s>>>(node_depth, ni, tiles4, map4);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    // deepest leaf = two levels below the deepest internal node (root at 0, leaves counted)
-    if (depth_out) *depth_out = maxd + 2;
-    if (n4_out) *n4_out = n4;
-    return hipStreamSynchronize(s);
+    return refit_levels(b, L, s, depth_out, n4_out);
 }
 
 hipError_t bvh4_collapse(const LbvhBuffers& b, float4* nodes4, hipStream_t s) {

@@ -29,8 +29,13 @@ size_t lbvh_scratch_bytes(uint32_t n);
 // Returns 0 or a HIP error; writes max leaf depth to *depth_out and the number
 // of 4-wide nodes to *n4_out (host) when not null.
 hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out, uint32_t* n4_out);
-// After lbvh_build with the same buffers (scratch still holding depths and the
-// 4-wide numbering): writes the n4 x 128 B 4-wide nodes.
+// Refit of a committed tree: b.prim (slot -> triangle) and b.nodes (topology,
+// parent in field 14) are kept; the triangles are re-gathered from b.tri_pos /
+// tri_nrm / tri_uv / tri_mat and every box recomputed bottom-up, then the
+// 4-wide numbering is rebuilt in scratch for bvh4_collapse.
+hipError_t lbvh_refit(const LbvhBuffers& b, hipStream_t s);
+// After lbvh_build or lbvh_refit with the same buffers (scratch still holding
+// depths and the 4-wide numbering): writes the n4 x 128 B 4-wide nodes.
 hipError_t bvh4_collapse(const LbvhBuffers& b, float4* nodes4, hipStream_t s);
 
 // Standalone stable radix sort of 32-bit keys with 32-bit values (exposed for tests).

@@ -73,6 +73,8 @@ SIGNATURES = [
     ("ohpt_scene_tree_depth", C.c_int, [_vp, _u32p]),
     ("ohpt_scene_export_bvh", C.c_int, [_vp, _vp, _vp, _u32p]),
     ("ohpt_scene_export_bvh4", C.c_int, [_vp, _vp, C.c_uint32, _u32p]),
+    ("ohpt_scene_set_mesh_transform", C.c_int, [_vp, C.c_uint32, _f32p]),
+    ("ohpt_scene_refit", C.c_int, [_vp]),
     ("ohpt_upload_materials", C.c_int, [_vp, _vp, C.c_uint32]),
     ("ohpt_upload_texture", C.c_int, [_vp, C.POINTER(C.c_int32), _f32p, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("ohpt_update_texture", C.c_int, [_vp, C.c_int32, _f32p, C.c_int]),
@@ -230,6 +232,15 @@ class Context:
 
     def commit_scene(self):
         _check(self.L.ohpt_scene_commit(self.h), "ohpt_scene_commit")
+
+    def set_mesh_transform(self, mesh: int, world=None):
+        """New 4x4 world matrix (row-major numpy, like add_mesh) of mesh `mesh`; applied by refit()."""
+        w = None if world is None else np.ascontiguousarray(np.asarray(world, np.float32).T.reshape(-1))
+        _check(self.L.ohpt_scene_set_mesh_transform(self.h, mesh, _fp(w)), "ohpt_scene_set_mesh_transform")
+
+    def refit(self):
+        """Refits the committed tree to the current mesh transforms (no rebuild)."""
+        _check(self.L.ohpt_scene_refit(self.h), "ohpt_scene_refit")
 
     def tree_depth(self) -> int:
         d = C.c_uint32()

@@ -114,6 +114,14 @@ void oracle_scene_commit(void* h) {
     s.build_emitters();
 }
 void oracle_scene_rebuild_emitters(void* h) { static_cast<Scene*>(h)->build_emitters(); }
+// New world-space positions (n x 9, caller's order) and optionally normals; refits the tree.
+void oracle_scene_refit(void* h, const float* pos, const float* nrm) {
+    Scene& s = *static_cast<Scene*>(h);
+    std::memcpy(s.pos.data(), pos, (size_t)s.n * 36);
+    if (nrm && !s.nrm.empty()) std::memcpy(s.nrm.data(), nrm, (size_t)s.n * 36);
+    lbvh_refit(s.pos.data(), s.n, s.bvh);
+    s.build_emitters();
+}
 
 uint32_t oracle_scene_bvh(void* h, void* nodes_out, float* tris_out, uint32_t* prim_out,
                           uint64_t* keys_out) {

@@ -2,6 +2,7 @@
 #include "oracle_core.hpp"
 
 #include <limits>
+#include <stdexcept>
 #include <numeric>
 
 namespace oracle {
@@ -104,6 +105,8 @@ static void tri_box(const float* t, float lo[3], float hi[3]) {
     }
 }
 
+static void refit_boxes(Bvh& out);
+
 void lbvh_build(const float* tp, uint32_t n, Bvh& out) {
     out = Bvh{};
     out.n = n;
@@ -178,7 +181,14 @@ void lbvh_build(const float* tp, uint32_t n, Bvh& out) {
                 if (!(nd.child[c] & kLeafBit)) out.nodes[nd.child[c]].parent = (uint32_t)i;
         }
     }
-    // Bottom-up refit, post-order (children before parents), root = node 0.
+    refit_boxes(out);
+    bvh4_collapse(out);
+}
+
+// Bottom-up box refit over the committed topology, post-order (children
+// before parents), root = node 0.
+static void refit_boxes(Bvh& out) {
+    const uint32_t ni = (uint32_t)out.nodes.size();
     std::vector<uint32_t> post;
     post.reserve(ni);
     {
@@ -214,7 +224,21 @@ void lbvh_build(const float* tp, uint32_t n, Bvh& out) {
             for (int a = 0; a < 3; ++a) { nd.b[c * 6 + a] = lo[a]; nd.b[c * 6 + 3 + a] = hi[a]; }
         }
     }
-    bvh4_collapse(out);
+}
+
+void lbvh_refit(const float* tp, uint32_t n, Bvh& bvh) {
+    if (n != bvh.n) throw std::runtime_error("lbvh_refit: triangle count changed");
+    if (n == 0) return;
+    for (uint32_t s = 0; s < n; ++s) {
+        const uint32_t p = bvh.prim[s];
+        const float* t = tp + (size_t)p * 9;
+        float* d = &bvh.tri[(size_t)s * 12];
+        d[0] = t[0]; d[1] = t[1]; d[2] = t[2];
+        d[4] = t[3]; d[5] = t[4]; d[6] = t[5];
+        d[8] = t[6]; d[9] = t[7]; d[10] = t[8];
+    }
+    refit_boxes(bvh);
+    bvh4_collapse(bvh);
 }
 
 // ------------------------------------------------------------------ FP32 traversal

@@ -187,6 +187,9 @@ struct Bvh {
 
 uint32_t morton30(float cx, float cy, float cz, const float cmin[3], const float cmax[3]);
 void lbvh_build(const float* tri_pos /* n*9 */, uint32_t n, Bvh& out);   // binary LBVH + 4-wide collapse
+// Refit: same topology and leaf order, triangles re-read from tri_pos (caller's
+// order), boxes recomputed bottom-up, 4-wide nodes re-collapsed.
+void lbvh_refit(const float* tri_pos /* n*9 */, uint32_t n, Bvh& bvh);
 // Collapse (called by lbvh_build): every internal node at even depth becomes a
 // 4-wide node, numbered in binary-node order; its slots are its binary
 // children in order, an internal child replaced by that child's two children.

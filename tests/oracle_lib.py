@@ -58,6 +58,7 @@ def lib():
         L.oracle_scene_set_settings.argtypes = [vp, vp]
         L.oracle_scene_commit.argtypes = [vp]
         L.oracle_scene_rebuild_emitters.argtypes = [vp]
+        L.oracle_scene_refit.argtypes = [vp, f32p, f32p]
         L.oracle_scene_bvh.argtypes = [vp, vp, f32p, u32p, u64p]
         L.oracle_scene_bvh.restype = C.c_uint32
         L.oracle_tri_hit.argtypes = [f32p, f32p, f32p]
@@ -159,6 +160,13 @@ class OracleScene:
         self.L.oracle_scene_bvh(self.h, nodes.ctypes.data, _p(tris), _p(prim, C.c_uint32),
                                 _p(keys, C.c_uint64))
         return nodes, tris, prim, keys
+
+    def refit(self, pos, nrm=None):
+        """New world-space triangles (n, 9) in the scene's order; same tree topology."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 9)
+        self._refit_keep = pos
+        n = None if nrm is None else np.ascontiguousarray(nrm, np.float32).reshape(-1, 9)
+        self.L.oracle_scene_refit(self.h, _p(pos), None if n is None else _p(n))
 
     def bvh4(self):
         """4-wide nodes as (n4, 32) uint32: lo.x[4] lo.y[4] lo.z[4] hi.x[4] hi.y[4] hi.z[4] ref[4] meta[4]."""

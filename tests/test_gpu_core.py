@@ -124,6 +124,51 @@ def test_watertight_rounded_zero_edges(oracle):
     ctx.close()
 
 
+def test_refit_after_mesh_transform(oracle):
+    """Three meshes; one is moved with ohpt_scene_set_mesh_transform and the tree
+    refitted: binary and 4-wide nodes, leaf order and triangles equal the
+    oracle's refit of the same world positions bit for bit, the topology is the
+    committed one, and traversal agrees with the oracle."""
+    rng = np.random.default_rng(31)
+    meshes = []
+    for k in range(3):
+        v = (rng.uniform(-3, 3, (900, 3)) + np.array([6.0 * (k - 1), 0, 0])).astype(np.float32)
+        idx = np.arange(900, dtype=np.uint32)
+        meshes.append((v, idx))
+    move = np.eye(4, dtype=np.float32)
+    move[:3, :3] *= 2.0
+    move[:3, 3] = [0.5, 1.25, -0.75]
+    ctx = ohpt.Context(8, 8)
+    ctx.begin_scene()
+    for v, idx in meshes:
+        ctx.add_mesh(v, idx)
+    ctx.commit_scene()
+    n = 900
+    nodes0, _, prim0 = ctx.export_bvh(n)
+    ctx.set_mesh_transform(1, move)
+    ctx.refit()
+    g_nodes, g_tris, g_prim = ctx.export_bvh(n)
+    g4 = ctx.export_bvh4()
+    before = np.concatenate([v[idx].reshape(-1, 9) for v, idx in meshes]).astype(np.float32)
+    after = before.copy().reshape(-1, 3, 3)
+    after[300:600] = after[300:600] * np.float32(2.0) + np.float32([0.5, 1.25, -0.75])   # exact in FP32
+    after = after.reshape(-1, 9)
+    o = oracle.OracleScene(plain_scene(before))
+    o.refit(after)
+    c_nodes, c_tris, c_prim, _ = o.bvh()
+    assert np.array_equal(g_prim, prim0) and np.array_equal(g_nodes[:, 12:14], nodes0[:, 12:14])
+    assert np.array_equal(g_prim, c_prim)
+    assert np.array_equal(g_tris.view(np.uint32), c_tris.view(np.uint32))
+    assert np.array_equal(g_nodes[:, :14], c_nodes[:, :14])
+    assert np.array_equal(g4, o.bvh4())
+    rays = random_rays(8000, 41, origin_box=10.0)
+    g = ctx.trace_rays(rays)
+    c, _, _ = o.trace(rays)
+    assert np.array_equal(g.view(np.uint32), c.view(np.uint32))
+    log(f"refit: {n} triangles, {int((g[:, 1].view(np.uint32) != 0xFFFFFFFF).sum())} hits of 8000 rays")
+    ctx.close()
+
+
 def test_trace_tmin_tmax_window(oracle):
     sc = SCENES["soup1000"]()
     rays = random_rays(5000, 21)

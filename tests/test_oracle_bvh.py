@@ -169,6 +169,31 @@ def test_lbvh_duplicate_centroids(oracle):
     _check_bvh4(o.bvh4(), nodes, 64)
 
 
+def test_refit_keeps_topology_and_matches_brute_force(oracle):
+    """Moving part of the geometry and refitting keeps the topology and leaf
+    order, rebuilds every box exactly (the Karras invariants hold on the moved
+    triangles) and traversal still finds the brute-force closest hit."""
+    n = 1500
+    pos = random_soup(n, seed=11)
+    sc = S.Scene(pos=pos, mat=np.zeros(n, np.uint32), materials=np.array([S.lambert((0.5,) * 3)]),
+                 camera=S.Camera.look_at((0, 0, 20), (0, 0, 0), 40), width=4, height=4)
+    o = oracle.OracleScene(sc)
+    nodes0, _, prim0, keys0 = o.bvh()
+    moved = pos.copy().reshape(n, 3, 3)
+    moved[n // 2:] = moved[n // 2:] * np.float32(2.0) + np.float32([1.5, -0.75, 0.25])
+    moved = moved.reshape(n, 9)
+    o.refit(moved)
+    nodes1, tris1, prim1, keys1 = o.bvh()
+    assert np.array_equal(nodes1[:, 12:14], nodes0[:, 12:14]) and np.array_equal(prim1, prim0)
+    _check_tree(nodes1, tris1, prim1, keys1, moved)
+    _check_bvh4(o.bvh4(), nodes1, n)
+    rays = random_rays(3000, 23, origin_box=8.0)
+    hits, _, _ = o.trace(rays)
+    brute = o.brute(rays)
+    bad, _ = compare_to_brute(moved, hits[:, 1].view(np.uint32), hits[:, 0], brute, rays)
+    assert bad == []
+
+
 def random_rays(n, seed, origin_box=6.0, tmin=0.0):
     rng = np.random.default_rng(seed)
     o = rng.uniform(-origin_box, origin_box, (n, 3))
