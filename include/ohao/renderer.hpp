@@ -93,6 +93,9 @@ public:
     // Material / light edits without a BVH rebuild.
     [[nodiscard]] bool updateRTMaterialParams();
     [[nodiscard]] bool updateRTLightParams();
+    // Actor transform edits (the TLAS instance update of the reference): the
+    // committed BVH is refitted, not rebuilt. Lights follow updateRTLightParams.
+    [[nodiscard]] bool updateRTTransforms();
     // Texel edits of scene textures (e.g. after an inverse-rendering step).
     [[nodiscard]] bool updateTexture(int32_t id);
 

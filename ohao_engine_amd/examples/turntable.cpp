@@ -72,7 +72,7 @@ std::vector<float> sky(uint32_t w, uint32_t h) {
 int main(int argc, char** argv) {
     if (argc < 3) {
         std::printf("Usage: turntable <model.obj|sphere> <cornell|dark|env|mirror> [spp] [frames] "
-                    "[--env=file.hdr] [--size=WxH] [--out=dir] [--denoise=learned]\n");
+                    "[--env=file.hdr] [--size=WxH] [--out=dir] [--denoise=learned] [--spin|--spin-rebuild]\n");
         return 1;
     }
     const std::string model_path = argv[1], mode = argv[2];
@@ -80,12 +80,16 @@ int main(int argc, char** argv) {
     uint32_t W = 1280, H = 720;
     std::string env_path, out_dir = "renders/turntable";
     DenoiseMode dmode = DenoiseMode::None;
+    bool spin = false;      // move the model (refit, no rebuild) instead of orbiting the camera
+    bool rebuild = false;   // --spin-rebuild: the same motion through a full scene rebuild
     for (int i = 3; i < argc; ++i) {
         const std::string a = argv[i];
         if (a.rfind("--env=", 0) == 0) env_path = a.substr(6);
         else if (a.rfind("--size=", 0) == 0) std::sscanf(a.c_str() + 7, "%ux%u", &W, &H);
         else if (a.rfind("--out=", 0) == 0) out_dir = a.substr(6);
         else if (a.rfind("--denoise=", 0) == 0) dmode = parseDenoiseMode(a.substr(10));
+        else if (a == "--spin") spin = true;
+        else if (a == "--spin-rebuild") spin = rebuild = true;
         else if (pos == 0) { spp = std::max(1, std::atoi(a.c_str())); ++pos; }
         else if (pos == 1) { frames = std::max(1, std::atoi(a.c_str())); ++pos; }
     }
@@ -117,6 +121,8 @@ int main(int argc, char** argv) {
             wall(scene, "Back", LBB, RBB, RTB, LTB, {0, 0, 1}, white);
         }
     }
+    std::shared_ptr<Actor> model_actor;
+    vec3 model_pos(0.0f);
     auto model = std::make_shared<Model>();
     bool loaded = false;
     if (model_path == "sphere") {
@@ -140,8 +146,10 @@ int main(int argc, char** argv) {
         const vec3 c = (bmin + bmax) * 0.5f;
         const float floor_y = (mode == "env") ? 0.0f : -S;
         auto actor = scene.createActor("Model");
+        model_actor = actor;
         actor->getTransform()->setScale(vec3(scale));
-        actor->getTransform()->setPosition({-c.x * scale, floor_y - bmin.y * scale, -c.z * scale});
+        model_pos = vec3(-c.x * scale, floor_y - bmin.y * scale, -c.z * scale);
+        actor->getTransform()->setPosition(model_pos);
         auto mc = actor->addComponent<MeshComponent>();
         mc->setModel(model);
         Material& m = actor->addComponent<MaterialComponent>()->getMaterial();
@@ -186,11 +194,22 @@ int main(int argc, char** argv) {
     ::mkdir(out_dir.c_str(), 0755);
     for (int f = 0; f < frames; ++f) {
         const float ang = (float)f / (float)frames * 6.2831853f;
-        const float cx = radius * std::cos(ang), cz = radius * std::sin(ang);
+        const float cang = spin ? 0.0f : ang;
+        const float cx = radius * std::cos(cang), cz = radius * std::sin(cang);
         auto& cam = renderer.getCamera();
         cam.setPosition({cx, height, cz});
         cam.setRotation(15.0f, degrees(std::atan2(-cz, -cx)));
         cam.setFov(70.0f);
+        if (spin && f > 0) {
+            auto* tr = model_actor->getTransform();
+            tr->setRotation(quat(vec3(0.0f, ang, 0.0f)));
+            tr->setPosition(model_pos + vec3(1.5f * std::sin(ang), 0.0f, 0.0f));
+            if (rebuild) renderer.setScene(&scene);
+            if (!(rebuild ? renderer.updateSceneBuffers() : renderer.updateRTTransforms())) {
+                std::fprintf(stderr, "FATAL: frame %d: %s\n", f, renderer.lastError().c_str());
+                return 1;
+            }
+        }
         renderer.resetAccumulation();
         for (int s = 0; s < spp + 3; ++s) renderer.render();
         char name[512];

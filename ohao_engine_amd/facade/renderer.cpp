@@ -258,6 +258,21 @@ bool Renderer::updateRTMaterialParams() {
     return true;
 }
 
+bool Renderer::updateRTTransforms() {
+    if (!ctx_ || !scene_ || scene_dirty_) return false;
+    uint32_t mesh = 0;
+    for (const auto& [id, a] : scene_->getAllActors()) {   // the order and filter of updateSceneBuffers
+        auto mc = a->getComponent<MeshComponent>();
+        if (!mc || !mc->isVisible() || !mc->getModel() || mc->getModel()->indices.empty()) continue;
+        const mat4 world = a->getTransform()->getWorldMatrix();
+        if (ohpt_scene_set_mesh_transform(ctx_, mesh++, world.data()) != OHPT_OK)
+            return fail("ohpt_scene_set_mesh_transform");
+    }
+    if (ohpt_scene_refit(ctx_) != OHPT_OK) return fail("ohpt_scene_refit");
+    resetAccumulation();
+    return true;
+}
+
 bool Renderer::updateRTLightParams() {
     if (!ctx_ || !scene_ || scene_dirty_) return false;
     if (!upload_lights()) return false;

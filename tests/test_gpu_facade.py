@@ -60,6 +60,63 @@ def test_turntable_driver_writes_frames(tmp_path):
     assert len(frames) == 2 and all(f.stat().st_size > 96 * 64 * 4 for f in frames)
 
 
+def read_png_rgba8(path):
+    """8-bit RGBA, non-interlaced PNG (the façade's writePNG) -> (h, w, 4) uint8."""
+    import struct
+    import zlib
+    data = Path(path).read_bytes()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w = 8, b"", 0
+    while pos < len(data):
+        n, kind = struct.unpack(">I4s", data[pos:pos + 8])
+        chunk = data[pos + 8:pos + 8 + n]
+        if kind == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", chunk[:10])
+            assert depth == 8 and ctype == 6
+        elif kind == b"IDAT":
+            idat += chunk
+        pos += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + 4 * w)
+    out = np.zeros((h, 4 * w), np.int32)
+    for y in range(h):
+        f, row = raw[y, 0], raw[y, 1:].astype(np.int32)
+        prev = out[y - 1] if y else np.zeros(4 * w, np.int32)
+        cur = np.zeros(4 * w, np.int32)
+        for x in range(4 * w):
+            a = cur[x - 4] if x >= 4 else 0
+            b, c = prev[x], (prev[x - 4] if x >= 4 else 0)
+            if f == 0: pr = 0
+            elif f == 1: pr = a
+            elif f == 2: pr = b
+            elif f == 3: pr = (a + b) // 2
+            else:
+                pp = a + b - c
+                pa, pb, pc = abs(pp - a), abs(pp - b), abs(pp - c)
+                pr = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+            cur[x] = (row[x] + pr) & 255
+        out[y] = cur
+    return out.reshape(h, w, 4).astype(np.uint8)
+
+
+def test_turntable_spin_refits_instead_of_rebuilding(tmp_path):
+    """--spin moves the model (rotation + translation) with updateRTTransforms
+    (set_mesh_transform + refit of the committed BVH) under a fixed camera;
+    --spin-rebuild makes the same motion through a full scene rebuild. The
+    frames change where the model moved, and the refitted frame is the rebuilt
+    one (same random numbers; the two trees may break exact ties differently)."""
+    args = ["sphere", "cornell", 8, 3, "--size=96x64"]
+    run([BIN / "turntable", *args, "--spin", f"--out={tmp_path / 'refit'}"], tmp_path)
+    run([BIN / "turntable", *args, "--spin-rebuild", f"--out={tmp_path / 'rebuild'}"], tmp_path)
+    fr = [read_png_rgba8(p) for p in sorted((tmp_path / "refit").glob("cornell_*.png"))]
+    fb = [read_png_rgba8(p) for p in sorted((tmp_path / "rebuild").glob("cornell_*.png"))]
+    assert len(fr) == len(fb) == 3
+    moved = (np.abs(fr[0].astype(int) - fr[1].astype(int)).sum(-1) > 24).mean()
+    same = [(np.abs(a.astype(int) - b.astype(int)).sum(-1) > 24).mean() for a, b in zip(fr, fb)]
+    print(f"spin: {moved:.3f} of the pixels changed; refit vs rebuild differing fraction {same}")
+    assert moved > 0.02
+    assert max(same) < 0.005
+
+
 def test_learned_denoise_mode_through_facade(tmp_path):
     weights = Path(ohpt.__file__).resolve().parent / "weights" / "kpn_unet.bin"
     if not weights.exists():
