@@ -48,11 +48,14 @@ def trace_bytes(rays, visits, tests):
 BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak
 
 
-def equal_spp(plain, target):
-    """Plain spp whose relMSE equals `target`: log-log interpolation between the
-    measured checkpoints, slope -1 (MSE ~ 1/spp) outside them."""
+def equal_spp(plain, target, key="relmse"):
+    """Plain spp whose error `key` equals `target`: log-log interpolation
+    between the measured checkpoints, slope -1 (MSE ~ 1/spp; for an RMSE key
+    the squared values are interpolated) outside them."""
     xs = sorted(plain)
-    ls = [math.log(plain[c]["relmse"]) for c in xs]
+    sq = 2.0 if key.endswith("rmse") and key != "relmse" else 1.0
+    ls = [sq * math.log(plain[c][key]) for c in xs]
+    target = target ** sq
     lt = math.log(target)
     if lt >= ls[0]:
         return xs[0] * math.exp(ls[0] - lt)
@@ -115,9 +118,12 @@ def denoise_leg(ctx, sc, ref_spp, save_dir=None):
     untrained = ohpt.Denoiser(seed=1)
     ctx.set_render_seed(1)
     ctx.reset()
+    D.with_guides(ctx, sc.settings)   # the 1-spp input carries the first-hit guides
     plain, done, gpu_ms = {}, 0, 0.0
     den = None
     for c in (1, 4, 16, 64, 256, 1024):
+        if done == 1:
+            ctx.set_settings(sc.settings)   # plain accumulation records no guides
         ctx.render(c - done)
         gpu_ms += ctx.stats().ms_total
         done = c
@@ -141,6 +147,17 @@ def denoise_leg(ctx, sc, ref_spp, save_dir=None):
             gate = ld.validate((img[None], alb[None], nrm[None], dep[None]), ref)
             und = untrained.infer(img, alb, nrm, dep)[0]
             e_untrained, l_untrained = D.relmse(und, ref), ldr_rmse(und, ref)
+            # variance-guided a-trous comparator on the same 1-spp frame
+            atr = ctx.denoise_atrous()
+            import torch
+            atr_dev = torch.empty((sc.height, sc.width, 4), dtype=torch.float32, device="cuda:0")
+            times = []
+            for _ in range(5):
+                t1 = time.perf_counter()
+                ctx.denoise_atrous(out=atr_dev)   # returns after its stream synchronises
+                times.append((time.perf_counter() - t1) * 1e3)
+            ms_atr = float(np.median(times))
+            l_atr = float(np.sqrt(np.mean((atr[..., :3] - ldr(ref)) ** 2)))
             if save_dir:
                 save_png(Path(save_dir) / "denoised_1spp.png", den)
                 save_png(Path(save_dir) / "reference.png", ref)
@@ -149,6 +166,8 @@ def denoise_leg(ctx, sc, ref_spp, save_dir=None):
     untrained.close()
     e_den, l_den = D.relmse(den, ref), ldr_rmse(den, ref)
     spp_eq = equal_spp(plain, e_den)
+    spp_eq_ldr = equal_spp(plain, l_den, "ldr_rmse")
+    spp_eq_atr = equal_spp(plain, l_atr, "ldr_rmse")
     ms_per_spp = plain[1024]["gpu_ms"] / 1024.0
     t_den = plain[1]["gpu_ms"] + ms_den
     t_plain = spp_eq * ms_per_spp
@@ -167,6 +186,13 @@ def denoise_leg(ctx, sc, ref_spp, save_dir=None):
            "gpu_time_ratio_vs_plain_1024": round(plain[1024]["gpu_ms"] / t_den, 2),
            "plain_spp_equal": round(spp_eq, 2), "plain_spp_equal_extrapolated": bool(spp_eq > 1024),
            "gpu_ms_plain_equal": round(t_plain, 3), "speedup_at_equal_relmse": round(t_plain / t_den, 2),
+           "plain_spp_equal_ldr_rmse": round(spp_eq_ldr, 2),
+           "atrous": {"ldr_rmse": round(l_atr, 6), "ms_wall": round(ms_atr, 3),
+                      "plain_spp_equal_ldr_rmse": round(spp_eq_atr, 2),
+                      "gpu_ms_plain_equal": round(spp_eq_atr * ms_per_spp, 3),
+                      "speedup_at_equal_ldr_rmse": round(spp_eq_atr * ms_per_spp / (plain[1]["gpu_ms"] + ms_atr), 2),
+                      "note": "variance-guided a-trous (SVGF spatial pass) of the same 1-spp frame; its output is "
+                              "display-referred, so it is scored in LDR RMSE; ms_wall is host wall time of the call"},
            "gate": {"enabled": gate.enabled, "bound": gate.bound},
            "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 5),
@@ -175,6 +201,8 @@ def denoise_leg(ctx, sc, ref_spp, save_dir=None):
     log(f"denoise: relMSE denoised {e_den:.5f} (untrained {e_untrained:.5f}, 1 spp {plain[1]['relmse']:.5f}, "
         f"1024 spp {plain[1024]['relmse']:.5f}, reference halves {halves_relmse:.5f}); LDR RMSE denoised {l_den:.5f} "
         f"vs 1024 spp {plain[1024]['ldr_rmse']:.5f}; GPU {t_den:.2f} ms vs {plain[1024]['gpu_ms']:.1f} ms")
+    log(f"denoise: a-trous LDR RMSE {l_atr:.5f} (= plain {spp_eq_atr:.1f} spp), {ms_atr:.3f} ms; learned "
+        f"= plain {spp_eq_ldr:.1f} spp in LDR RMSE")
     return res
 
 

@@ -21,7 +21,8 @@ namespace ohao {
 enum class RenderMode { Forward, Deferred, RTRealtime, RTOffline };
 
 // denoise_types.hpp:13-19 plus Learned (the kernel-predicting network of
-// ohpt_denoiser_*). OIDN requests are served by Learned.
+// ohpt_denoiser_*). OIDN requests are served by Learned, Atrous by the
+// variance-guided a-trous filter; NRD and DLSSRR fall back to None.
 enum class DenoiseMode : uint32_t { None = 0, OIDN = 1, NRD = 3, Atrous = 4, DLSSRR = 5, Learned = 6 };
 DenoiseMode parseDenoiseMode(std::string_view s);
 const char* denoiseModeName(DenoiseMode m);
@@ -118,8 +119,9 @@ public:
     void setRenderSeed(uint32_t seed);
     void notifyCameraChanged() { resetAccumulation(); }
 
-    // RGBA8 (ACES(x*0.5))^(1/2.2); denoised when the denoise mode is on and
-    // the denoiser's quality gate is open.
+    // RGBA8 (ACES(x*0.5))^(1/2.2); with DenoiseMode::Learned/OIDN denoised
+    // when the denoiser's quality gate is open, with DenoiseMode::Atrous
+    // filtered by the variance-guided a-trous pass (ohpt_denoise_atrous).
     [[nodiscard]] const uint8_t* getPixels() const;
     [[nodiscard]] std::span<const uint8_t> getPixelSpan() const {
         const uint8_t* p = getPixels();

@@ -47,7 +47,8 @@ enum {
 
 /* ---- render flags ------------------------------------------------------ */
 enum {
-    OHPT_FLAG_AOVS = 1u << 0,          /* write first-hit albedo/normal/depth AOVs (PT_FLAG_ENABLE_AOVS) */
+    OHPT_FLAG_AOVS = 1u << 0,          /* write first-hit albedo/normal/depth AOVs (PT_FLAG_ENABLE_AOVS)
+                                          and the display-space sums (ohpt_read_display_sums) */
     OHPT_FLAG_FIREFLY_CLAMP = 1u << 2, /* luminance clamp per sample (PT_FLAG_ENABLE_FIREFLY_CLAMP) */
     OHPT_FLAG_NO_ENV_NEE = 1u << 8,    /* disable env importance sampling (BSDF sampling only) */
     OHPT_FLAG_NO_LIGHT_NEE = 1u << 9   /* disable light/emitter NEE (BSDF sampling only) */
@@ -239,6 +240,12 @@ int ohpt_read_accum(ohpt_context* ctx, float* out, int on_device);
  * per-pixel variance (no reference counterpart: the reference keeps only the
  * running mean). */
 int ohpt_read_moments(ohpt_context* ctx, float* sum, float* sumsq, int on_device);
+/* Display-space sums since the last reset, recorded with OHPT_FLAG_AOVS:
+ * (sum of the per-sample display value (ACES(x*0.5))^(1/2.2) per channel,
+ * sum of its squared luminance), RGBA32F: the colour and luminance moments
+ * the reference's SVGF temporal pass accumulates (rt_svgf_temporal.comp:87-99),
+ * read by ohpt_denoise_atrous. */
+int ohpt_read_display_sums(ohpt_context* ctx, float* out, int on_device);
 /* RGBA8: (ACES(x*0.5))^(1/2.2) as pt_raygen_offline.rgen:1338-1354. */
 int ohpt_read_pixels(ohpt_context* ctx, uint8_t* out, int on_device);
 /* First-hit AOVs (readbackHDRBuffers, render_dispatch.cpp:536-626):
@@ -338,6 +345,18 @@ int ohpt_denoiser_set_profiling(ohpt_denoiser* dn, int on);
 /* Denoise the context's current accumulation (mean beauty + first-hit AOVs)
  * into out (width*height RGBA32F of the context). */
 int ohpt_denoise(ohpt_context* ctx, ohpt_denoiser* dn, float* out, int on_device);
+
+/* Variance-guided a-trous filter (the SVGF spatial pass) of the context's
+ * current accumulation into out (width*height RGBA32F display values in
+ * [0,1], alpha 1): replaces AtrousDenoiser::dispatch
+ * (ohao/render/rt/denoise/atrous_denoise.hpp:59-60) with
+ * shaders/rt/rt_svgf_atrous.comp:49-115 over 5 iterations (steps 1..16,
+ * sigmaL 0.4, sigmaNormal 0.3, sigmaDepth 2.0, atrous_denoise.cpp:29-40).
+ * Input colour = the mean per-sample display value, variance = the
+ * per-sample display-luminance variance (raised to the 3x3 spatial variance
+ * below 4 samples, rt_svgf_temporal.comp:105-120), normal/depth = the
+ * first-hit AOVs (OHPT_FLAG_AOVS must be set, else OHPT_ERR_STATE). */
+int ohpt_denoise_atrous(ohpt_context* ctx, float* out, int on_device);
 
 #ifdef __cplusplus
 }

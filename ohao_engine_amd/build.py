@@ -19,7 +19,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libohpt.so"
 ARCH = os.environ.get("OHPT_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "lbvh.hip", "traverse.hip", "integrator.hip", "env.hip", "denoiser.hip"]
+SOURCES = ["api.hip", "lbvh.hip", "traverse.hip", "integrator.hip", "env.hip", "denoiser.hip", "atrous.hip"]
 HEADERS = ["ohpt_device.h", "ohpt_internal.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

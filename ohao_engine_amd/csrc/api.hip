@@ -97,7 +97,7 @@ struct ohpt_context {
     uint32_t seed = 0;
     uint32_t sample_index = 0;
 
-    DevBuf d_accum, d_moments, d_aov_albedo, d_aov_normal;
+    DevBuf d_accum, d_moments, d_aov_albedo, d_aov_normal, d_disp;
 
     // path state
     uint32_t capacity = 0;
@@ -162,6 +162,7 @@ int alloc_frame(ohpt_context* c) {
     HIPCHK(c->d_moments.ensure(npix * 16));
     HIPCHK(c->d_aov_albedo.ensure(npix * 16));
     HIPCHK(c->d_aov_normal.ensure(npix * 16));
+    HIPCHK(c->d_disp.ensure(npix * 16));
     return OHPT_OK;
 }
 
@@ -171,6 +172,7 @@ int clear_frame(ohpt_context* c) {
     launch_fill_u32(c->d_moments.as<uint32_t>(), 0u, (uint32_t)(npix * 4), c->stream);
     launch_fill_u32(c->d_aov_albedo.as<uint32_t>(), 0u, (uint32_t)(npix * 4), c->stream);
     launch_fill_u32(c->d_aov_normal.as<uint32_t>(), 0u, (uint32_t)(npix * 4), c->stream);
+    launch_fill_u32(c->d_disp.as<uint32_t>(), 0u, (uint32_t)(npix * 4), c->stream);
     HIPCHK(hipGetLastError());
     return OHPT_OK;
 }
@@ -946,7 +948,7 @@ int ohpt_render(ohpt_context* c, uint32_t spp, uint32_t render_flags) {
             }
         }
         launch_accumulate(f, ps, c->d_accum.as<float4>(), c->d_moments.as<float4>(), c->d_aov_albedo.as<float4>(),
-                          c->d_aov_normal.as<float4>(), c->stream);
+                          c->d_aov_normal.as<float4>(), c->d_disp.as<float4>(), c->stream);
         HIPCHK(hipGetLastError());
         if (!(render_flags & OHPT_RENDER_ASYNC)) {
             host_counters.resize((size_t)kCounterWords * nl);
@@ -1226,6 +1228,15 @@ int ohpt_read_moments(ohpt_context* c, float* sum, float* sumsq, int on_device) 
     return OHPT_OK;
 }
 
+int ohpt_read_display_sums(ohpt_context* c, float* out, int on_device) {
+    if (!c || !out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_read_display_sums: null argument");
+    HIPCHK(hipSetDevice(c->device));
+    const hipMemcpyKind k = on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    HIPCHK(hipMemcpyAsync(out, c->d_disp.p, (size_t)c->W * c->H * 16, k, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OHPT_OK;
+}
+
 int ohpt_read_pixels(ohpt_context* c, uint8_t* out, int on_device) {
     if (!c || !out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_read_pixels: null argument");
     const uint32_t n = c->W * c->H;
@@ -1283,6 +1294,20 @@ int ohpt_denoise(ohpt_context* c, ohpt_denoiser* dn, float* out, int on_device) 
     if (rc != OHPT_OK) return rc;
     if (!on_device) HIPCHK(hipMemcpy(out, dst, (size_t)n * 16, hipMemcpyDeviceToHost));
     return OHPT_OK;
+}
+
+int ohpt_denoise_atrous(ohpt_context* c, float* out, int on_device) {
+    if (!c || !out) return fail(OHPT_ERR_INVALID_ARG, "ohpt_denoise_atrous: null argument");
+    if (!(c->st.flags & OHPT_FLAG_AOVS))
+        return fail(OHPT_ERR_STATE, "ohpt_denoise_atrous: the edge stops need OHPT_FLAG_AOVS");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(c->dn_guides.ensure(svgf_scratch_bytes(c->W, c->H)));
+    const uint32_t n = c->W * c->H;
+    return produce(c, out, (size_t)n * 16, on_device, [&](void* dst) {
+        svgf_atrous(c->d_accum.as<float4>(), c->d_disp.as<float4>(), c->d_aov_albedo.as<float4>(),
+                    c->d_aov_normal.as<float4>(), c->W, c->H, reinterpret_cast<float4*>(dst), c->dn_guides.p,
+                    c->stream);
+    });
 }
 
 // ---------------------------------------------------------------- ray batches

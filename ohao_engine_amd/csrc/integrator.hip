@@ -431,17 +431,30 @@ __global__ __launch_bounds__(256) void k_adjoint_init(uint32_t pid_base, uint32_
     lrem[pid] = make_float4(a.x + b.x + c.x, a.y + b.y + c.y, a.z + b.z + c.z, 0.0f);
 }
 
+// (ACES(x*0.5))^(1/2.2) -> [0, 1], pt_raygen_offline.rgen:102-105, :1338-1354
+__device__ __forceinline__ float aces(float x) {
+    return fminf(1.0f, fmaxf(0.0f, (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f)));
+}
+__device__ __forceinline__ float display(float x) { return powf(aces(x * 0.5f), 1.0f / 2.2f); }
+
+// Per-pixel sums over this launch's samples. With AOVs on, also the first-hit
+// guides and the display-space sums the a-trous filter reads: sum of the
+// per-sample display value (RGB) and of its squared luminance (A), what the
+// reference's temporal pass accumulates frame by frame (rt_svgf_temporal.comp:
+// 87-99; one sample per frame there).
 __global__ __launch_bounds__(256) void k_accumulate(FrameParams f, const float4* __restrict__ L,
                                                     const float4* __restrict__ ln0, const float4* __restrict__ ln1,
                                                     const float4* __restrict__ aov0, const float4* __restrict__ aov1,
                                                     float4* __restrict__ accum, float4* __restrict__ moments,
-                                                    float4* __restrict__ aov_albedo, float4* __restrict__ aov_normal) {
+                                                    float4* __restrict__ aov_albedo, float4* __restrict__ aov_normal,
+                                                    float4* __restrict__ disp) {
     const uint32_t npix = f.width * f.height;
     const uint32_t pixel = blockIdx.x * 256 + threadIdx.x;
     if (pixel >= npix) return;
     const uint32_t spp = f.n_paths / npix;
     float sx = 0, sy = 0, sz = 0, qx = 0, qy = 0, qz = 0;
     float ax = 0, ay = 0, az = 0, dep = 0, nx = 0, ny = 0, nz = 0, nh = 0;
+    float tx = 0, ty = 0, tz = 0, tl2 = 0;
     const bool aovs = (f.flags & OHPT_FLAG_AOVS) != 0;
     for (uint32_t k = 0; k < spp; ++k) {
         const uint32_t pid = k * npix + pixel;
@@ -458,6 +471,9 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams f, const float4*
             const float4 p0 = aov0[pid], p1 = aov1[pid];
             ax += p0.x; ay += p0.y; az += p0.z; dep += p0.w;
             nx += p1.x; ny += p1.y; nz += p1.z; nh += p1.w;
+            const float dx = display(rx), dy = display(ry), dz = display(rz);
+            const float l = 0.2126f * dx + 0.7152f * dy + 0.0722f * dz;
+            tx += dx; ty += dy; tz += dz; tl2 += l * l;
         }
     }
     float4 acc = accum[pixel];
@@ -473,6 +489,9 @@ __global__ __launch_bounds__(256) void k_accumulate(FrameParams f, const float4*
         float4 an = aov_normal[pixel];
         an.x += nx; an.y += ny; an.z += nz; an.w += nh;
         aov_normal[pixel] = an;
+        float4 ds = disp[pixel];
+        ds.x += tx; ds.y += ty; ds.z += tz; ds.w += tl2;
+        disp[pixel] = ds;
     }
 }
 
@@ -484,10 +503,7 @@ __global__ __launch_bounds__(256) void k_resolve(const float4* __restrict__ acc,
     out[i] = make_float4(a.x * inv, a.y * inv, a.z * inv, a.w);
 }
 
-// (ACES(x*0.5))^(1/2.2) -> RGBA8, pt_raygen_offline.rgen:102-105, :1338-1354
-__device__ __forceinline__ float aces(float x) {
-    return fminf(1.0f, fmaxf(0.0f, (x * (2.51f * x + 0.03f)) / (x * (2.43f * x + 0.59f) + 0.14f)));
-}
+// display value of the mean -> RGBA8
 __global__ __launch_bounds__(256) void k_tonemap(const float4* __restrict__ acc, uint8_t* __restrict__ out, uint32_t n) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -497,7 +513,7 @@ __global__ __launch_bounds__(256) void k_tonemap(const float4* __restrict__ acc,
     uchar4 o;
     uint8_t v[3];
     for (int k = 0; k < 3; ++k) {
-        const float t = powf(aces(c[k] * 0.5f), 1.0f / 2.2f);
+        const float t = display(c[k]);
         v[k] = (uint8_t)fminf(255.0f, fmaxf(0.0f, t * 255.0f + 0.5f));
     }
     o.x = v[0]; o.y = v[1]; o.z = v[2]; o.w = 255;
@@ -567,10 +583,10 @@ void launch_adjoint_init(const PathState& ps, uint32_t pid_base, uint32_t n, flo
 }
 
 void launch_accumulate(const FrameParams& f, const PathState& ps, float4* accum, float4* moments,
-                       float4* aov_albedo, float4* aov_normal, hipStream_t s) {
+                       float4* aov_albedo, float4* aov_normal, float4* disp, hipStream_t s) {
     const uint32_t npix = f.width * f.height;
     k_accumulate<<<div_up(npix, 256), 256, 0, s>>>(f, ps.L, ps.ln0, ps.ln1, ps.aov0, ps.aov1, accum, moments,
-                                                   aov_albedo, aov_normal);
+                                                   aov_albedo, aov_normal, disp);
 }
 
 void launch_resolve(const float4* accum, float4* out, uint32_t n, hipStream_t s) {

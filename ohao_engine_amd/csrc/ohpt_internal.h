@@ -146,7 +146,7 @@ hipError_t compact_records(const uint32_t* count, uint32_t nv, const uint32_t* k
 // lrem[pid] = L + light-NEE + env-NEE radiance for pid in [pid_base, pid_base + n)
 void launch_adjoint_init(const PathState& ps, uint32_t pid_base, uint32_t n, float4* lrem, hipStream_t s);
 void launch_accumulate(const FrameParams& f, const PathState& ps, float4* accum, float4* moments,
-                       float4* aov_albedo, float4* aov_normal, hipStream_t s);
+                       float4* aov_albedo, float4* aov_normal, float4* disp, hipStream_t s);
 void launch_resolve(const float4* accum, float4* out, uint32_t n, hipStream_t s);
 void launch_tonemap(const float4* accum, uint8_t* out, uint32_t n, hipStream_t s);
 void launch_resolve_aovs(const float4* accum, const float4* aov_albedo, const float4* aov_normal,
@@ -162,6 +162,19 @@ hipError_t env_cdf_build(const float4* env, uint32_t W, uint32_t H, float* marg,
                          hipStream_t s);
 // Environment importance samples for inspection: u (n x 2) -> out (n x 4: dir.xyz, pdf; pdf 0 = no sample).
 void launch_env_sample(const DevScene& S, const float* u, float4* out, uint32_t n, hipStream_t s);
+
+// ---- variance-guided a-trous comparator (atrous.hip) ----------------------------
+// The reference's SVGF spatial pass on the accumulated frame (mean display
+// colour, display-luminance variance, first-hit AOVs): out = W*H RGBA32F
+// display values in [0,1].
+constexpr int kSvgfIterations = 5;           // steps 1, 2, 4, 8, 16 (atrous_denoise.cpp:29)
+constexpr float kSvgfSigmaL = 0.4f;          // atrous_denoise.cpp:38-40
+constexpr float kSvgfSigmaN = 0.30f;
+constexpr float kSvgfSigmaD = 2.0f;
+size_t svgf_scratch_bytes(uint32_t W, uint32_t H);
+hipError_t svgf_atrous(const float4* accum, const float4* disp, const float4* aov_albedo,
+                       const float4* aov_normal, uint32_t W, uint32_t H, float4* out, void* scratch,
+                       hipStream_t s);
 
 // ---- denoiser (denoiser.hip) ---------------------------------------------------
 // Inference on device-resident guides (RGBA32F radiance/albedo/normal, F32

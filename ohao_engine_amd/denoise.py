@@ -16,7 +16,7 @@ import json
 import math
 import sys
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from pathlib import Path
 
 import numpy as np
@@ -68,6 +68,13 @@ class Sample:
     target: object = None
 
 
+def with_guides(ctx: ohpt.Context, settings: S.Settings):
+    """Apply `settings` with first-hit AOV recording on: the albedo, normal
+    and depth guides the network reads (and the a-trous filter's edge stops)
+    are written only under OHPT_FLAG_AOVS."""
+    ctx.set_settings(replace(settings, flags=settings.flags | S.FLAG_AOVS))
+
+
 def render_view(ctx: ohpt.Context, cam, n_inputs: int, target_spp: int, seed: int, torch, device) -> Sample:
     H, W = ctx.height, ctx.width
     smp = Sample()
@@ -99,6 +106,7 @@ def generate(scene_seeds, views_per_scene: int, size: int, n_inputs: int, target
         sc = S.procedural(width=size, height=size, seed=seed)
         ctx = ohpt.Context(size, size)
         ctx.load_scene(sc)
+        with_guides(ctx, sc.settings)
         rng = np.random.default_rng(seed * 7919 + 1)
         for v in range(views_per_scene):
             data.append(render_view(ctx, random_view(rng), n_inputs, target_spp, 1 + 97 * v, torch, device))

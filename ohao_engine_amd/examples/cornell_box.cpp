@@ -9,7 +9,7 @@
 // (0.63, 0.065, 0.05), green (0.14, 0.45, 0.091). Camera at (0.278, 0.273,
 // -0.8), yaw 90, fov 39.3; depth 8, Russian roulette from bounce 3, PCG32.
 //
-//   cornell_box [out.png] [spp=64] [--size=WxH] [--seed=N] [--hdr=out.pfm] [--denoise=learned]
+//   cornell_box [out.png] [spp=1024] [--size=WxH (1920x1080)] [--seed=N] [--hdr=out.pfm] [--denoise=learned|atrous]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -90,8 +90,8 @@ bool write_pfm(const std::string& path, const std::vector<float>& rgba, uint32_t
 
 int main(int argc, char** argv) {
     std::string out = "cornell_box.png", hdr;
-    int spp = 64;
-    uint32_t W = 256, H = 256, seed = 1;
+    int spp = 1024;                        // cornell_box.cpp:71-73: 1024 spp, 1920x1080
+    uint32_t W = 1920, H = 1080, seed = 1;
     DenoiseMode dmode = DenoiseMode::None;
     int pos = 0;
     for (int i = 1; i < argc; ++i) {

@@ -5,7 +5,7 @@
 // resetAccumulation, render() x (spp + 3), getPixelSpan.
 //
 //   turntable <model.obj|sphere> <cornell|dark|env|mirror> [spp=64] [frames=120]
-//             [--env=sky.hdr] [--size=WxH] [--out=dir] [--denoise=learned]
+//             [--env=sky.hdr] [--size=WxH] [--out=dir] [--denoise=learned|atrous]
 #include <sys/stat.h>
 
 #include <cmath>
@@ -72,7 +72,7 @@ std::vector<float> sky(uint32_t w, uint32_t h) {
 int main(int argc, char** argv) {
     if (argc < 3) {
         std::printf("Usage: turntable <model.obj|sphere> <cornell|dark|env|mirror> [spp] [frames] "
-                    "[--env=file.hdr] [--size=WxH] [--out=dir] [--denoise=learned] [--spin|--spin-rebuild]\n");
+                    "[--env=file.hdr] [--size=WxH] [--out=dir] [--denoise=learned|atrous] [--spin|--spin-rebuild]\n");
         return 1;
     }
     const std::string model_path = argv[1], mode = argv[2];

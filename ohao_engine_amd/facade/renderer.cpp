@@ -384,6 +384,18 @@ const uint8_t* Renderer::getPixels() const {
             for (int k = 0; k < 3; ++k) pixels_[i * 4 + k] = tonemap8(den[i * 4 + k]);
             pixels_[i * 4 + 3] = 255;
         }
+    } else if (denoise_ == DenoiseMode::Atrous && samples_ > 0) {
+        // the a-trous comparator filters display values: quantise as the tonemap does
+        std::vector<float> den(n * 4);
+        if (ohpt_denoise_atrous(ctx_, den.data(), 0) != OHPT_OK) {
+            fail("ohpt_denoise_atrous");
+            return nullptr;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            for (int k = 0; k < 3; ++k)
+                pixels_[i * 4 + k] = (uint8_t)std::min(255.0f, std::max(0.0f, den[i * 4 + k] * 255.0f + 0.5f));
+            pixels_[i * 4 + 3] = 255;
+        }
     } else if (ohpt_read_pixels(ctx_, pixels_.data(), 0) != OHPT_OK) {
         fail("ohpt_read_pixels");
         return nullptr;
@@ -436,12 +448,14 @@ bool Renderer::loadDenoiserWeights(const std::string& path) {
 
 void Renderer::setDenoiseMode(DenoiseMode mode) {
     pixels_valid_ = false;
-    if (mode == DenoiseMode::NRD || mode == DenoiseMode::Atrous || mode == DenoiseMode::DLSSRR) {
+    if (mode == DenoiseMode::NRD || mode == DenoiseMode::DLSSRR) {
         err_ = std::string("setDenoiseMode: ") + denoiseModeName(mode) + " is not provided; using None";
         denoise_ = DenoiseMode::None;
         return;
     }
     denoise_ = mode;
+    // the a-trous filter needs the first-hit AOVs, which the renderer always records
+    if (mode == DenoiseMode::Atrous) return;
     if (mode != DenoiseMode::None && !ensure_denoiser()) denoise_ = DenoiseMode::None;
 }
 

@@ -117,6 +117,8 @@ SIGNATURES = [
     ("ohpt_denoiser_get_stats", C.c_int, [_vp, C.POINTER(DenoiserStats)]),
     ("ohpt_denoiser_set_profiling", C.c_int, [_vp, C.c_int]),
     ("ohpt_denoise", C.c_int, [_vp, _vp, _f32p, C.c_int]),
+    ("ohpt_denoise_atrous", C.c_int, [_vp, _f32p, C.c_int]),
+    ("ohpt_read_display_sums", C.c_int, [_vp, _f32p, C.c_int]),
 ]
 
 _lib = None
@@ -365,6 +367,13 @@ class Context:
         _check(self.L.ohpt_read_moments(self.h, _fp(s), _fp(q), 0), "ohpt_read_moments")
         return s, q
 
+    def read_display_sums(self):
+        """(H, W, 4): per-pixel sums of the display value (RGB) and of its
+        squared luminance (A), recorded with FLAG_AOVS."""
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        _check(self.L.ohpt_read_display_sums(self.h, _fp(out), 0), "ohpt_read_display_sums")
+        return out
+
     def read_pixels(self) -> np.ndarray:
         out = np.zeros((self.height, self.width, 4), np.uint8)
         _check(self.L.ohpt_read_pixels(self.h, out.ctypes.data_as(_u8p), 0), "ohpt_read_pixels")
@@ -390,6 +399,17 @@ class Context:
             return out
         res = np.zeros((self.height, self.width, 4), np.float32)
         _check(self.L.ohpt_denoise(self.h, dn.h, _fp(res), 0), "ohpt_denoise")
+        return res
+
+    def denoise_atrous(self, out=None):
+        """Variance-guided a-trous filter (SVGF spatial pass) of the current
+        accumulation -> (H, W, 4) display values in [0, 1] (numpy), or into
+        the device tensor `out`."""
+        if out is not None and hasattr(out, "data_ptr"):
+            _check(self.L.ohpt_denoise_atrous(self.h, C.cast(out.data_ptr(), _f32p), 1), "ohpt_denoise_atrous")
+            return out
+        res = np.zeros((self.height, self.width, 4), np.float32)
+        _check(self.L.ohpt_denoise_atrous(self.h, _fp(res), 0), "ohpt_denoise_atrous")
         return res
 
     # -- differentiation ---------------------------------------------------------

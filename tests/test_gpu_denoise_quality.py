@@ -32,6 +32,7 @@ def heldout():
     sc = S.procedural(width=size, height=size, seed=4242)
     ctx = ohpt.Context(size, size)
     ctx.load_scene(sc)
+    D.with_guides(ctx, sc.settings)
     ctx.set_camera(D.random_view(np.random.default_rng(4242)))
     ctx.set_render_seed(7_000_000)
     ctx.reset()

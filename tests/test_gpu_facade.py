@@ -126,6 +126,20 @@ def test_learned_denoise_mode_through_facade(tmp_path):
     assert (tmp_path / "dn.png").stat().st_size > 64 * 64 * 4
 
 
+def test_atrous_denoise_mode_through_facade(tmp_path):
+    out = run([BIN / "cornell_box", tmp_path / "plain.png", 4, "--size=64x48", "--seed=3"], tmp_path)
+    out_a = run([BIN / "cornell_box", tmp_path / "atrous.png", 4, "--size=64x48", "--seed=3", "--denoise=atrous"],
+                tmp_path)
+    assert "Denoise mode: Atrous" in out_a and "Denoise mode: None" in out
+    plain = read_png_rgba8(tmp_path / "plain.png").astype(np.int32)
+    filt = read_png_rgba8(tmp_path / "atrous.png").astype(np.int32)
+    assert plain.shape == filt.shape == (48, 64, 4)
+    # the filter smooths: neighbour differences shrink, the mean level stays
+    tv = lambda im: np.abs(np.diff(im[..., :3], axis=1)).mean()
+    assert tv(filt) < 0.7 * tv(plain)
+    assert abs(filt[..., :3].mean() - plain[..., :3].mean()) < 8
+
+
 def test_inverse_fit_driver_selftest(tmp_path):
     out = run([BIN / "inverse_fit", "--selftest"], tmp_path, timeout=300)
     assert "Done:" in out
