@@ -289,11 +289,22 @@ int build_emitters(ohpt_context* c) {
                      cz = e1[0] * e2[1] - e1[1] * e2[0];
         const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
         if (!(area > 0)) continue;
+        // vertex normals define the emitting side: store the triangle with
+        // vertices 1 and 2 swapped when the normals oppose the winding
+        bool sw = false;
+        if (!c->h_nrm.empty()) {
+            const float* q = &c->h_nrm[(size_t)i * 9];
+            const double nx = (double)q[0] + q[3] + q[6], ny = (double)q[1] + q[4] + q[7],
+                         nz = (double)q[2] + q[5] + q[8];
+            sw = nx * nx + ny * ny + nz * nz > 1e-24 && cx * nx + cy * ny + cz * nz < 0;
+        }
+        const float* v1 = t + (sw ? 6 : 3);
+        const float* v2 = t + (sw ? 3 : 6);
         float mb;
         std::memcpy(&mb, &mi, 4);
         rec.push_back(make_float4(t[0], t[1], t[2], (float)area));
-        rec.push_back(make_float4(t[3], t[4], t[5], mb));
-        rec.push_back(make_float4(t[6], t[7], t[8], 0.0f));
+        rec.push_back(make_float4(v1[0], v1[1], v1[2], mb));
+        rec.push_back(make_float4(v2[0], v2[1], v2[2], 0.0f));
         w.push_back(area * l);
         total += area * l;
     }

@@ -80,7 +80,11 @@ __device__ __forceinline__ SurfaceHit surface(const DevScene& S, uint32_t slot, 
     f3 ng = normalize(cross(v1 - v0, v2 - v0));
     const f3 n0 = mk(a0.x, a0.y, a0.z), n1 = mk(a0.w, a1.x, a1.y), n2 = mk(a1.z, a1.w, a2.x);
     const f3 ni = n0 * w + n1 * b1 + n2 * b2;
-    f3 ns = dot(ni, ni) > 1e-12f ? normalize(ni) : ng;
+    const bool has_n = dot(ni, ni) > 1e-12f;
+    f3 ns = has_n ? normalize(ni) : ng;
+    // authored vertex normals define the outside (the reference shades with the
+    // interpolated normal whatever the winding, pt_closesthit.rchit:60-82)
+    if (has_n && dot(ng, ni) < 0.0f) ng = -ng;
     s.u = a2.y * w + a2.w * b1 + a3.y * b2;
     s.v = a2.z * w + a3.x * b1 + a3.z * b2;
     s.mat = __float_as_uint(a3.w);

@@ -1,15 +1,25 @@
 // Cornell box driver over the C++ façade. Same flow as the reference's
-// examples/cornell_box.cpp:64-179: VulkanRenderer(W,H) -> initialize() ->
+// examples/cornell_box.cpp:70-179: VulkanRenderer(W,H) -> initialize() ->
 // setScene -> camera -> resetAccumulation -> render() x (spp + 3) ->
 // getPixelSpan() -> PNG.
 //
-// Scene: the 0.555 m room (12 wall triangles plus a closed front wall), short
-// box rotated -17 deg, tall box rotated 15 deg, 2-triangle ceiling light with
-// emission (17, 12, 4); Lambert albedos white (0.725, 0.71, 0.68), red
-// (0.63, 0.065, 0.05), green (0.14, 0.45, 0.091). Camera at (0.278, 0.273,
-// -0.8), yaw 90, fov 39.3; depth 8, Russian roulette from bounce 3, PCG32.
+// Default scene = the reference driver's (cornell_box.cpp:84-144): a 10 m
+// open-front room (white 0.73, red (0.65, 0.05, 0.05) left, green
+// (0.12, 0.45, 0.15) right, roughness 0.95), a metal sphere (0.95, 0.93,
+// 0.88, roughness 0.05) and a smooth dielectric sphere (0.9, 0.95, 1.0,
+// roughness 0.02) built from the unit UV-sphere primitive, twelve coloured
+// sphere lights (intensity 5, radius 0.3); camera at (0, 0, 13), yaw -90,
+// fov 38; the façade's default settings (depth 4, Sobol).
 //
-//   cornell_box [out.png] [spp=1024] [--size=WxH (1920x1080)] [--seed=N] [--hdr=out.pfm] [--denoise=learned|atrous]
+// --scene=classic: the 0.555 m Cornell box (12 wall triangles plus a closed
+// front wall), short box rotated -17 deg, tall box rotated 15 deg, 2-triangle
+// ceiling light with emission (17, 12, 4); Lambert albedos white
+// (0.725, 0.71, 0.68), red (0.63, 0.065, 0.05), green (0.14, 0.45, 0.091).
+// Camera at (0.278, 0.273, -0.8), yaw 90, fov 39.3; depth 8, Russian roulette
+// from bounce 3, PCG32.
+//
+//   cornell_box [out.png] [spp=1024] [--size=WxH (1920x1080)] [--scene=reference|classic] [--seed=N]
+//               [--hdr=out.pfm] [--denoise=none|learned|atrous]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -76,41 +86,59 @@ void add_actor(Scene& scene, const char* name, const std::vector<Tri>& tris, vec
     mat.emissive = emission;
 }
 
-bool write_pfm(const std::string& path, const std::vector<float>& rgba, uint32_t w, uint32_t h) {
-    std::ofstream f(path, std::ios::binary);
-    if (!f) return false;
-    f << "PF\n" << w << " " << h << "\n-1.0\n";
-    for (uint32_t y = 0; y < h; ++y)
-        for (uint32_t x = 0; x < w; ++x)
-            f.write(reinterpret_cast<const char*>(&rgba[((size_t)(h - 1 - y) * w + x) * 4]), 12);
-    return (bool)f;
+// cornell_box.cpp:84-144
+void build_reference_scene(Scene& scene) {
+    const float S = 5.0f;
+    const vec3 white(0.73f), red(0.65f, 0.05f, 0.05f), green(0.12f, 0.45f, 0.15f);
+    const vec3 LBB(-S, -S, -S), RBB(S, -S, -S), LTB(-S, S, -S), RTB(S, S, -S);
+    const vec3 LBF(-S, -S, S), RBF(S, -S, S), LTF(-S, S, S), RTF(S, S, S);
+    auto wall = [&](const char* name, vec3 a, vec3 b, vec3 c, vec3 d, vec3 n, vec3 col) {
+        auto actor = scene.createActor(name);
+        actor->addTag("cornell_wall");
+        auto mc = actor->addComponent<MeshComponent>();
+        mc->setModel(makeQuad(a, b, c, d, n));
+        mc->setVisible(true);
+        Material& m = actor->addComponent<MaterialComponent>()->getMaterial();
+        m.baseColor = col;
+        m.roughness = 0.95f;
+    };
+    wall("Back", LBB, RBB, RTB, LTB, {0, 0, 1}, white);
+    wall("Left", LBB, LTB, LTF, LBF, {1, 0, 0}, red);
+    wall("Right", RBB, RBF, RTF, RTB, {-1, 0, 0}, green);
+    wall("Floor", LBB, LBF, RBF, RBB, {0, 1, 0}, white);
+    wall("Ceiling", LTB, RTB, RTF, LTF, {0, -1, 0}, white);
+
+    auto metal = scene.createActorWithComponents("MetalSphere", PrimitiveType::Sphere);
+    metal->getTransform()->setPosition({-2.0f, -S + 2.0f, 0.0f});
+    metal->getTransform()->setScale(vec3(2.0f));
+    Material& m1 = metal->getComponent<MaterialComponent>()->getMaterial();
+    m1.baseColor = {0.95f, 0.93f, 0.88f};
+    m1.roughness = 0.05f;
+    m1.metallic = 1.0f;
+    auto glass = scene.createActorWithComponents("GlassSphere", PrimitiveType::Sphere);
+    glass->getTransform()->setPosition({2.5f, -S + 1.8f, 1.5f});
+    glass->getTransform()->setScale(vec3(1.8f));
+    Material& m2 = glass->getComponent<MaterialComponent>()->getMaterial();
+    m2.baseColor = {0.9f, 0.95f, 1.0f};
+    m2.roughness = 0.02f;
+
+    const vec3 colors[12] = {{1.0f, 0.3f, 0.2f}, {0.2f, 1.0f, 0.3f}, {0.3f, 0.4f, 1.0f}, {1.0f, 0.9f, 0.3f},
+                             {1.0f, 0.5f, 0.0f}, {0.8f, 0.2f, 1.0f}, {0.0f, 1.0f, 1.0f}, {1.0f, 0.0f, 0.5f},
+                             {1.0f, 1.0f, 1.0f}, {0.5f, 1.0f, 0.5f}, {1.0f, 0.7f, 0.5f}, {0.5f, 0.7f, 1.0f}};
+    const float pos[12][3] = {{-3, 4, -3}, {0, 4, -3}, {3, 4, -3}, {-3, 4, 0}, {0, 4, 0}, {3, 4, 0},
+                              {-3, 4, 3},  {0, 4, 3},  {3, 4, 3},  {-4, 0, 0}, {4, 0, 0}, {0, 0, -4}};
+    for (int i = 0; i < 12; ++i) {
+        auto l = scene.createActor("Light" + std::to_string(i));
+        auto lc = l->addComponent<LightComponent>();
+        lc->setLightType(LightType::Sphere);
+        lc->setColor(colors[i]);
+        lc->setIntensity(5.0f);
+        lc->setRadius(0.3f);
+        l->getTransform()->setPosition({pos[i][0], pos[i][1], pos[i][2]});
+    }
 }
 
-}  // namespace
-
-int main(int argc, char** argv) {
-    std::string out = "cornell_box.png", hdr;
-    int spp = 1024;                        // cornell_box.cpp:71-73: 1024 spp, 1920x1080
-    uint32_t W = 1920, H = 1080, seed = 1;
-    DenoiseMode dmode = DenoiseMode::None;
-    int pos = 0;
-    for (int i = 1; i < argc; ++i) {
-        const std::string a = argv[i];
-        if (a.rfind("--size=", 0) == 0) std::sscanf(a.c_str() + 7, "%ux%u", &W, &H);
-        else if (a.rfind("--seed=", 0) == 0) seed = (uint32_t)std::strtoul(a.c_str() + 7, nullptr, 10);
-        else if (a.rfind("--hdr=", 0) == 0) hdr = a.substr(6);
-        else if (a.rfind("--denoise=", 0) == 0) dmode = parseDenoiseMode(a.substr(10));
-        else if (pos == 0) { out = a; ++pos; }
-        else if (pos == 1) { spp = std::max(1, std::atoi(a.c_str())); ++pos; }
-    }
-    std::printf("ohao cornell_box: %ux%u @ %d spp\n", W, H, spp);
-
-    VulkanRenderer renderer(W, H);
-    if (!renderer.initialize()) {
-        std::fprintf(stderr, "FATAL: renderer init failed: %s\n", renderer.lastError().c_str());
-        return 1;
-    }
-    Scene scene("Cornell Box");
+void build_classic_scene(Scene& scene) {
     const double R = 0.555, zf = -0.81;
     auto P = [](double x, double y, double z) { return vec3((float)x, (float)y, (float)z); };
     const vec3 white(0.725f, 0.71f, 0.68f), red(0.63f, 0.065f, 0.05f), green(0.14f, 0.45f, 0.091f), none(0.0f);
@@ -132,17 +160,67 @@ int main(int argc, char** argv) {
     const double ly = R - 1e-3;
     t.clear(); quad(t, P(0.213, ly, 0.227), P(0.343, ly, 0.227), P(0.343, ly, 0.332), P(0.213, ly, 0.332));
     add_actor(scene, "Light", t, vec3(0.78f), vec3(17.0f, 12.0f, 4.0f));
+}
 
+bool write_pfm(const std::string& path, const std::vector<float>& rgba, uint32_t w, uint32_t h) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return false;
+    f << "PF\n" << w << " " << h << "\n-1.0\n";
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x)
+            f.write(reinterpret_cast<const char*>(&rgba[((size_t)(h - 1 - y) * w + x) * 4]), 12);
+    return (bool)f;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string out = "cornell_box.png", hdr, scene_name = "reference";
+    int spp = 1024;                        // cornell_box.cpp:71-73: 1024 spp, 1920x1080
+    uint32_t W = 1920, H = 1080, seed = 1;
+    DenoiseMode dmode = DenoiseMode::None;
+    int pos = 0;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        if (a.rfind("--size=", 0) == 0) std::sscanf(a.c_str() + 7, "%ux%u", &W, &H);
+        else if (a.rfind("--seed=", 0) == 0) seed = (uint32_t)std::strtoul(a.c_str() + 7, nullptr, 10);
+        else if (a.rfind("--hdr=", 0) == 0) hdr = a.substr(6);
+        else if (a.rfind("--scene=", 0) == 0) scene_name = a.substr(8);
+        else if (a.rfind("--denoise=", 0) == 0) dmode = parseDenoiseMode(a.substr(10));
+        else if (pos == 0) { out = a; ++pos; }
+        else if (pos == 1) { spp = std::max(1, std::atoi(a.c_str())); ++pos; }
+    }
+    std::printf("ohao cornell_box: %ux%u @ %d spp\n", W, H, spp);
+
+    VulkanRenderer renderer(W, H);
+    if (!renderer.initialize()) {
+        std::fprintf(stderr, "FATAL: renderer init failed: %s\n", renderer.lastError().c_str());
+        return 1;
+    }
+    if (scene_name != "reference" && scene_name != "classic") {
+        std::fprintf(stderr, "FATAL: unknown --scene=%s (reference|classic)\n", scene_name.c_str());
+        return 1;
+    }
+    const bool classic = scene_name == "classic";
+    Scene scene("Cornell Box");
+    if (classic) build_classic_scene(scene);
+    else build_reference_scene(scene);
     renderer.setScene(&scene);
-    RTRenderSettings st;
-    st.maxBounces = 8;
-    st.rrStart = 3;
-    st.sobol = false;
-    renderer.setRTRenderSettings(st);
     auto& cam = renderer.getCamera();
-    cam.setPosition({0.278f, 0.273f, -0.8f});
-    cam.setRotation(0.0f, 90.0f);
-    cam.setFov(39.3f);
+    if (classic) {
+        RTRenderSettings st;
+        st.maxBounces = 8;
+        st.rrStart = 3;
+        st.sobol = false;
+        renderer.setRTRenderSettings(st);
+        cam.setPosition({0.278f, 0.273f, -0.8f});
+        cam.setRotation(0.0f, 90.0f);
+        cam.setFov(39.3f);
+    } else {
+        cam.setPosition({0.0f, 0.0f, 13.0f});
+        cam.setFov(38.0f);
+        cam.setRotation(0.0f, -90.0f);
+    }
     renderer.setRenderMode(RenderMode::RTOffline);
     if (dmode != DenoiseMode::None) renderer.setDenoiseMode(dmode);
     std::printf("Denoise mode: %s\n", denoiseModeName(renderer.getDenoiseMode()));

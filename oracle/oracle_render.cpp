@@ -309,7 +309,10 @@ Surf<R> interact(const Scene& sc, const Hit<R>& h, V3<R> d) {
     if (!sc.nrm.empty()) {
         const float* np = &sc.nrm[(size_t)h.prim * 9];
         const V3<R> n = v3<R>(np) * w + v3<R>(np + 3) * h.u + v3<R>(np + 6) * h.v;
-        if (dot(n, n) > R(1e-12)) ns = normalize(n);
+        if (dot(n, n) > R(1e-12)) {
+            ns = normalize(n);
+            if (dot(ng, n) < 0) ng = -ng;   // vertex normals define the outside (pt_closesthit.rchit:60-82)
+        }
     }
     if (!sc.uv.empty()) {
         const float* up = &sc.uv[(size_t)h.prim * 6];
@@ -485,7 +488,8 @@ V3<R> path_sample(const Scene& sc, uint32_t px, uint32_t py, uint32_t sidx) {
                 const size_t ei = std::min<size_t>((size_t)(it - sc.emit_cdf.begin()), sc.emit_tri.size() - 1);
                 const uint32_t et = sc.emit_tri[ei];
                 const float* tp = &sc.pos[(size_t)et * 9];
-                const V3<R> a0 = v3<R>(tp), a1 = v3<R>(tp + 3), a2 = v3<R>(tp + 6);
+                const bool sw = sc.emit_swap[ei] != 0;   // vertices 1, 2 swapped: winding follows the normals
+                const V3<R> a0 = v3<R>(tp), a1 = v3<R>(tp + (sw ? 6 : 3)), a2 = v3<R>(tp + (sw ? 3 : 6));
                 const R su = std::sqrt(R(vs.u[1]));
                 const R b0 = R(1) - su, b1 = R(vs.u[2]) * su, b2 = R(1) - b0 - b1;
                 const V3<R> x = a0 * b0 + a1 * b1 + a2 * b2;
@@ -560,6 +564,7 @@ V3<R> path_sample(const Scene& sc, uint32_t px, uint32_t py, uint32_t sidx) {
 
 void Scene::build_emitters() {
     emit_tri.clear();
+    emit_swap.clear();
     emit_cdf.clear();
     emit_total = 0.0;
     std::vector<double> w;
@@ -571,7 +576,14 @@ void Scene::build_emitters() {
         const V3<double> a0 = v3<double>(t), a1 = v3<double>(t + 3), a2 = v3<double>(t + 6);
         const double area = 0.5 * length(cross(a1 - a0, a2 - a0));
         if (!(area > 0)) continue;
+        bool sw = false;
+        if (!nrm.empty()) {
+            const float* q = &nrm[(size_t)i * 9];
+            const V3<double> ns = v3<double>(q) + v3<double>(q + 3) + v3<double>(q + 6);
+            sw = dot(ns, ns) > 1e-24 && dot(cross(a1 - a0, a2 - a0), ns) < 0;
+        }
         emit_tri.push_back(i);
+        emit_swap.push_back(sw ? 1 : 0);
         w.push_back(area * l);
         emit_total += area * l;
     }

@@ -40,6 +40,7 @@ struct Scene {
     float env_integral = 0.0f;
     // emitter triangles
     std::vector<uint32_t> emit_tri;
+    std::vector<uint8_t> emit_swap;      // 1: sample with vertices 1 and 2 swapped (normals oppose the winding)
     std::vector<double> emit_cdf;        // normalised, size = emit_tri.size()
     double emit_total = 0.0;             // sum of area * lum(Le)
     ohpt_camera cam{};

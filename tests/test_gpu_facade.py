@@ -37,7 +37,7 @@ def read_pfm(path):
 
 def test_cornell_box_driver_matches_binding(tmp_path):
     spp, size = 16, 64
-    out = run([BIN / "cornell_box", tmp_path / "cb.png", spp, f"--size={size}x{size}", "--seed=1",
+    out = run([BIN / "cornell_box", tmp_path / "cb.png", spp, f"--size={size}x{size}", "--seed=1", "--scene=classic",
                f"--hdr={tmp_path / 'cb.pfm'}"], tmp_path)
     assert f"{spp} samples per pixel" in out
     png = (tmp_path / "cb.png").read_bytes()
@@ -121,14 +121,15 @@ def test_learned_denoise_mode_through_facade(tmp_path):
     weights = Path(ohpt.__file__).resolve().parent / "weights" / "kpn_unet.bin"
     if not weights.exists():
         pytest.skip("no trained denoiser weights in the package")
-    out = run([BIN / "cornell_box", tmp_path / "dn.png", 4, "--size=64x64", "--denoise=learned"], tmp_path)
+    out = run([BIN / "cornell_box", tmp_path / "dn.png", 4, "--size=64x64", "--scene=classic", "--denoise=learned"], tmp_path)
     assert "Denoise mode: Learned" in out
     assert (tmp_path / "dn.png").stat().st_size > 64 * 64 * 4
 
 
 def test_atrous_denoise_mode_through_facade(tmp_path):
-    out = run([BIN / "cornell_box", tmp_path / "plain.png", 4, "--size=64x48", "--seed=3"], tmp_path)
-    out_a = run([BIN / "cornell_box", tmp_path / "atrous.png", 4, "--size=64x48", "--seed=3", "--denoise=atrous"],
+    out = run([BIN / "cornell_box", tmp_path / "plain.png", 4, "--size=64x48", "--seed=3", "--scene=classic"], tmp_path)
+    out_a = run([BIN / "cornell_box", tmp_path / "atrous.png", 4, "--size=64x48", "--seed=3", "--scene=classic",
+                 "--denoise=atrous"],
                 tmp_path)
     assert "Denoise mode: Atrous" in out_a and "Denoise mode: None" in out
     plain = read_png_rgba8(tmp_path / "plain.png").astype(np.int32)

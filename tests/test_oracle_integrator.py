@@ -106,6 +106,34 @@ def test_enclosure_closed_form(oracle, D, flags):
     assert np.all(np.abs(mean - want) <= 5.0 * np.sqrt(var / n) + 1e-6), (mean, want)
 
 
+def _reverse_winding(sc, inward_normals):
+    """The enclosure with every triangle's winding reversed (facing out);
+    optionally with flat vertex normals pointing in."""
+    import dataclasses
+    p = sc.pos.reshape(-1, 3, 3).astype(np.float64)
+    n_in = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    n_in /= np.linalg.norm(n_in, axis=1, keepdims=True)
+    pos = sc.pos.reshape(-1, 3, 3)[:, [0, 2, 1], :].reshape(-1, 9).copy()   # vertices 1 and 2 swapped
+    nrm = np.repeat(n_in[:, None, :], 3, axis=1).reshape(-1, 9).astype(np.float32) if inward_normals else None
+    return dataclasses.replace(sc, pos=pos, nrm=nrm)
+
+
+@pytest.mark.parametrize("flags", [0, S.FLAG_NO_LIGHT_NEE])
+def test_vertex_normals_define_the_outside(oracle, flags):
+    # Inward-facing walls given by winding, or by vertex normals against an
+    # outward winding, are the same enclosure: the hit side, the emitting side
+    # and the sampled emitters' orientation follow the normals. Without the
+    # normals the outward winding shows the camera only back faces: black.
+    sc = S.enclosure((1.0, 0.5, 0.25), 0.5, max_bounces=2, width=12, height=12)
+    st = S.Settings(max_bounces=2, rr_start=1000, flags=flags, seed=8)
+    a, _ = oracle.OracleScene(sc, st).render(16, 0, precision=64)
+    b, _ = oracle.OracleScene(_reverse_winding(sc, True), st).render(16, 0, precision=64)
+    c, _ = oracle.OracleScene(_reverse_winding(sc, False), st).render(16, 0, precision=64)
+    assert a.min() > 0
+    np.testing.assert_allclose(b, a, rtol=1e-6)
+    assert c.max() == 0.0
+
+
 def test_russian_roulette_is_unbiased(oracle):
     Le = (1.0, 1.0, 1.0)
     sc = S.enclosure(Le, 0.6, max_bounces=6, width=12, height=12)
