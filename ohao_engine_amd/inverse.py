@@ -15,6 +15,13 @@ adjoint.
 
     python -m ohao_engine_amd.inverse fit   --iters 40 --report fit.json
     python -m ohao_engine_amd.inverse check --texels 64 --report check.json
+    python -m ohao_engine_amd.inverse heavy --iters 3 --report heavy.json
+
+`heavy` is the combined heavy case: 256 displaced spheres of 40,500
+triangles (10.37M), 2048x2048 albedo, roughness and metallic textures fitted
+from a seeded random init, 16 views at 1024x1024, 8 spp, depth 8, NEE + MIS on
+the procedural sky; each iteration renders every view, runs its adjoint and
+the learned denoiser on it, and takes one Adam step on the device.
 """
 from __future__ import annotations
 
@@ -205,21 +212,163 @@ def run_check(fit: InverseFit, n_texels: int, eps: float, seed: int = 0):
             "cosine": float(a @ f / (np.linalg.norm(a) * np.linalg.norm(f))), "rows": rows}
 
 
+class HeavyCase:
+    """The combined heavy workload on device-resident targets, gradients and
+    Adam state (torch tensors on cuda:0; the library runs on its own stream and
+    every call returns after that stream synchronises)."""
+
+    def __init__(self, size=1024, spp=8, views=16, tex=2048, freq=45, instances=256, target_spp=32, seed=5,
+                 denoise=True):
+        import torch
+        from . import denoise as D
+        self.torch = torch
+        self.dev = torch.device("cuda:0")
+        t0 = time.time()
+        self.gt = S.heavy_textures(tex, 32, seed)
+        self.sc = S.heavy(size, size, spp, seed, freq, instances, tex, textures=self.gt)
+        log(f"heavy scene: {self.sc.num_triangles} triangles generated in {time.time() - t0:.1f}s")
+        self.ctx = ohpt.Context(size, size)
+        self.ctx.load_scene(self.sc)
+        self.build_ms = self.ctx.stats().ms_build
+        D.with_guides(self.ctx, self.sc.settings)
+        side = int(round(instances ** 0.5))
+        self.cams = S.heavy_cameras(views, side)
+        self.spp, self.npix, self.size = spp, size * size, size
+        self.seeds = [2000 + v for v in range(views)]
+        t0 = time.time()
+        self.targets = []
+        for v, cam in enumerate(self.cams):
+            self._render(cam, 9_000_000 + 10_000 * v, target_spp)
+            t = torch.empty((size, size, 4), dtype=torch.float32, device=self.dev)
+            self.ctx.read_accum_device(t)
+            self.targets.append(t)
+        log(f"{views} targets of {size}x{size} at {target_spp} spp in {time.time() - t0:.1f}s")
+        rng = np.random.default_rng(seed + 1)
+        self.theta = [torch.from_numpy(rng.uniform(0.3, 0.7, g.shape).astype(np.float32)).to(self.dev)
+                      for g in self.gt]
+        self.m = [torch.zeros_like(t) for t in self.theta]
+        self.v = [torch.zeros_like(t) for t in self.theta]
+        self.grad = [torch.zeros_like(t) for t in self.theta]
+        for tid, t in enumerate(self.theta):
+            self.ctx.update_texture(tid, t)
+        self.img = torch.empty((size, size, 4), dtype=torch.float32, device=self.dev)
+        self.den = torch.empty_like(self.img)
+        self.dn = None
+        if denoise and D.WEIGHTS.exists():
+            self.dn = D.LearnedDenoiser(D.WEIGHTS)
+        self.step = 0
+
+    def _render(self, cam, seed, spp):
+        self.ctx.set_camera(cam)
+        self.ctx.set_render_seed(seed)
+        self.ctx.reset()
+        self.ctx.render(spp)
+
+    def texel_rmse(self):
+        return [float(((t - self.torch.from_numpy(g).to(self.dev)) ** 2).mean().sqrt()) for t, g in
+                zip(self.theta, self.gt)]
+
+    def iteration(self, lr=0.02, b1=0.9, b2=0.999, eps=1e-8):
+        """Forward, adjoint and denoiser on every view, then one Adam step
+        (AdamState::step's update, on the device) with the texels projected to
+        [0, 1]. Returns the loss and the GPU milliseconds of each stage."""
+        torch = self.torch
+        self.ctx.zero_gradients()
+        fwd = adj = dn_ms = 0.0
+        L = 0.0
+        for cam, seed, tgt in zip(self.cams, self.seeds, self.targets):
+            seed = seed + 100_003 * self.step
+            self._render(cam, seed, self.spp)
+            fwd += self.ctx.stats().ms_total
+            self.ctx.read_accum_device(self.img)
+            d = self.img[..., :3] - tgt[..., :3]
+            L += float((d * d).sum()) / (3.0 * self.npix)
+            dl = torch.zeros_like(self.img)
+            dl[..., :3] = 2.0 * d / (3.0 * self.npix)
+            torch.cuda.synchronize()
+            if self.dn is not None:
+                self.ctx.denoise(self.dn.dn, out=self.den)
+                dn_ms += self.dn.dn.stats().ms_total
+            self.ctx.set_render_seed(seed)
+            self.ctx.adjoint(dl, self.spp)
+            adj += self.ctx.stats().ms_total
+        self.step += 1
+        for tid in range(len(self.theta)):
+            self.ctx.texture_gradient_device(tid, self.grad[tid])
+            g = self.grad[tid]
+            self.m[tid].mul_(b1).add_(g, alpha=1 - b1)
+            self.v[tid].mul_(b2).addcmul_(g, g, value=1 - b2)
+            mh = self.m[tid] / (1 - b1 ** self.step)
+            vh = self.v[tid] / (1 - b2 ** self.step)
+            self.theta[tid].sub_(lr * mh / (vh.sqrt() + eps)).clamp_(0.0, 1.0)
+        torch.cuda.synchronize()
+        for tid, t in enumerate(self.theta):
+            self.ctx.update_texture(tid, t)
+        return {"loss": L, "forward_ms": fwd, "adjoint_ms": adj, "denoise_ms": dn_ms}
+
+    def close(self):
+        if self.dn is not None:
+            self.dn.close()
+        self.ctx.close()
+
+
+def run_heavy(a):
+    h = HeavyCase(a.size, a.spp, a.views, a.tex, a.freq, a.instances, a.target_spp)
+    paths = a.views * h.npix * a.spp
+    hist = []
+    t0 = time.time()
+    for it in range(a.iters):
+        t1 = time.time()
+        r = h.iteration(lr=a.lr)
+        r["wall_s"] = time.time() - t1
+        r["texel_rmse"] = h.texel_rmse()
+        hist.append(r)
+        log(f"heavy iter {it + 1}/{a.iters}: loss {r['loss']:.5f} forward {r['forward_ms']:.1f} ms "
+            f"({paths / (r['forward_ms'] * 1e3):.0f} Mpaths/s), adjoint {r['adjoint_ms']:.1f} ms, denoise "
+            f"{r['denoise_ms']:.1f} ms; wall {r['wall_s']:.2f}s ({time.time() - t0:.0f}s)")
+    last = hist[-1]
+    rep = {"workload": f"heavy: {h.sc.num_triangles} triangles ({a.instances} displaced spheres, freq {a.freq}) + "
+                       f"ground, 3 textures {a.tex}x{a.tex} (albedo RGB, roughness, metallic), {a.views} views "
+                       f"{a.size}x{a.size}, {a.spp} spp, depth 8, NEE+MIS, sky+sun env",
+           "triangles": h.sc.num_triangles, "lbvh_build_ms": h.build_ms, "paths_per_iteration": paths,
+           "forward_mpaths_per_s": paths / (last["forward_ms"] * 1e3),
+           "adjoint_mpaths_per_s": paths / (last["adjoint_ms"] * 1e3),
+           "adjoint_cost_vs_forward": last["adjoint_ms"] / last["forward_ms"],
+           "denoise_ms_per_view": last["denoise_ms"] / a.views,
+           "iteration_gpu_ms": last["forward_ms"] + last["adjoint_ms"] + last["denoise_ms"],
+           "history": hist}
+    h.close()
+    return rep
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["fit", "check"])
-    ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--spp", type=int, default=16)
-    ap.add_argument("--views", type=int, default=8)
-    ap.add_argument("--tex", type=int, default=1024)
-    ap.add_argument("--freq", type=int, default=100)
-    ap.add_argument("--target-spp", type=int, default=256)
-    ap.add_argument("--iters", type=int, default=40)
+    ap.add_argument("cmd", choices=["fit", "check", "heavy"])
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--size", type=int, default=None)
+    ap.add_argument("--spp", type=int, default=None)
+    ap.add_argument("--views", type=int, default=None)
+    ap.add_argument("--tex", type=int, default=None)
+    ap.add_argument("--freq", type=int, default=None)
+    ap.add_argument("--target-spp", type=int, default=None)
+    ap.add_argument("--iters", type=int, default=None)
     ap.add_argument("--lr", type=float, default=0.02)
     ap.add_argument("--texels", type=int, default=64)
     ap.add_argument("--eps", type=float, default=1e-3)
     ap.add_argument("--report", default=None)
     a = ap.parse_args(argv)
+    # per-workload defaults: the lantern fit / check, or the heavy case
+    dflt = dict(size=1024, spp=8, views=16, tex=2048, freq=45, target_spp=32, iters=3) if a.cmd == "heavy" else \
+        dict(size=512, spp=16, views=8, tex=1024, freq=100, target_spp=256, iters=40)
+    for k, v in dflt.items():
+        if getattr(a, k) is None:
+            setattr(a, k, v)
+    if a.cmd == "heavy":
+        rep = run_heavy(a)
+        log(json.dumps({k: v for k, v in rep.items() if k != "history"}))
+        if a.report:
+            Path(a.report).write_text(json.dumps(rep, indent=1))
+        return 0
     fit = InverseFit(a.size, a.spp, a.views, a.tex, a.freq, target_spp=a.target_spp)
     rep = {"workload": f"lantern {fit.sc.num_triangles} triangles, {a.tex}x{a.tex} RGB albedo texture, "
                        f"{a.views} views {a.size}x{a.size}, {a.spp} spp, depth 4, L2 image loss",

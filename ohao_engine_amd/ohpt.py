@@ -427,6 +427,12 @@ class Context:
         g[..., :min(d.shape[-1], 4)] = d[..., :4]
         _check(self.L.ohpt_adjoint(self.h, _fp(g), 0, spp, flags), "ohpt_adjoint")
 
+    def texture_gradient_device(self, tid: int, out):
+        """dL/d(texel) of texture tid into the device tensor `out` (texel layout)."""
+        _check(self.L.ohpt_read_texture_gradient(self.h, tid, C.cast(out.data_ptr(), _f32p), 1),
+               "ohpt_read_texture_gradient")
+        return out
+
     def texture_gradient(self, tid: int, shape) -> np.ndarray:
         out = np.zeros(shape, np.float32)
         _check(self.L.ohpt_read_texture_gradient(self.h, tid, _fp(out), 0), "ohpt_read_texture_gradient")

@@ -386,6 +386,65 @@ def procedural(width=1920, height=1080, seed=7, sphere_freq=40, hf_cells=(1024, 
                  settings=Settings(max_bounces=4, rr_start=3, sampler=SAMPLER_PCG32, seed=seed, spp=1))
 
 
+def heavy_textures(size=2048, tiles=32, seed=5):
+    """Ground truth of the heavy case: albedo (RGB checker, U[0.1, 0.9] tiles),
+    roughness (1 channel, U[0.2, 0.9] tiles), metallic (1 channel, tiles of 0 or
+    1 with probability 0.3), tiles of size/tiles texels."""
+    rng = np.random.default_rng(seed)
+    t = size // tiles
+    up = lambda a: np.repeat(np.repeat(a, t, axis=0), t, axis=1).astype(np.float32)
+    albedo = up(rng.uniform(0.1, 0.9, (tiles, tiles, 3)))
+    rough = up(rng.uniform(0.2, 0.9, (tiles, tiles, 1)))
+    metal = up((rng.random((tiles, tiles, 1)) < 0.3).astype(np.float64))
+    return albedo, rough, metal
+
+
+def heavy(width=1024, height=1024, spp=8, seed=5, freq=45, n_instances=256, tex_size=2048, textures=None,
+          env_size=(2048, 1024)) -> Scene:
+    """Combined heavy case: n_instances displaced spheres of 20*freq^2 triangles
+    (freq 45: 40,500 each, 10,368,000 for 256) on a 16 x 16 grid over a ground
+    quad, one material whose albedo, roughness and metallic come from three
+    tex_size^2 textures (the fitted parameters), the procedural scene's
+    sky+sun environment; depth 8, NEE + MIS, Russian roulette from bounce 3."""
+    rng = np.random.default_rng(seed)
+    V, N, UV, F = displaced_sphere(freq, 0.1, seed)
+    side = int(round(math.sqrt(n_instances)))
+    pitch = 3.0
+    half = 0.5 * pitch * (side - 1)
+    pos_l, nrm_l, uv_l = [], [], []
+    for k in range(n_instances):
+        i, j = k % side, k // side
+        cx, cz = -half + pitch * i + rng.uniform(-0.3, 0.3), -half + pitch * j + rng.uniform(-0.3, 0.3)
+        sc = float(rng.uniform(0.8, 1.2))
+        p_, n_, u_ = flatten(V, N, UV, F, sc, rng.uniform(0, 2 * np.pi), (cx, sc, cz))
+        pos_l.append(p_); nrm_l.append(n_); uv_l.append(u_)
+    g = half + pitch
+    Pq = lambda x, y, z: np.array([x, y, z], np.float64)
+    ground = np.array(_quad(Pq(-g, 0.0, -g), Pq(-g, 0.0, g), Pq(g, 0.0, g), Pq(g, 0.0, -g)), np.float32)
+    pos = np.concatenate(pos_l + [ground])
+    nrm = np.concatenate(nrm_l + [np.tile(np.array([0, 1, 0] * 3, np.float32), (2, 1))])
+    uv = np.concatenate(uv_l + [np.zeros((2, 6), np.float32)])
+    n_sph = pos.shape[0] - 2
+    mat = np.concatenate([np.zeros(n_sph, np.uint32), np.ones(2, np.uint32)])
+    tex = list(heavy_textures(tex_size, 32, seed) if textures is None else textures)
+    mats = np.array([material((1.0, 1.0, 1.0), roughness=1.0, metallic=1.0, albedo_tex=0, roughness_tex=1,
+                              metallic_tex=2),
+                     material((0.5, 0.5, 0.5), roughness=0.9, metallic=0.0)], MATERIAL_DTYPE)
+    sun = (math.cos(math.radians(35)) * math.cos(math.radians(60)), math.sin(math.radians(35)),
+           math.cos(math.radians(35)) * math.sin(math.radians(60)))
+    env = sky_env(env_size[0], env_size[1], sun_dir=sun)
+    cam = heavy_cameras(1, side, pitch)[0]
+    return Scene(pos=pos, nrm=nrm, uv=uv, mat=mat, materials=mats, textures=tex, env=env, camera=cam,
+                 width=width, height=height, name="heavy",
+                 settings=Settings(max_bounces=8, rr_start=3, sampler=SAMPLER_PCG32, seed=seed, spp=spp))
+
+
+def heavy_cameras(n=16, side=16, pitch=3.0):
+    """n views on a 35-degree ring around the heavy case's grid."""
+    extent = 0.5 * pitch * (side - 1)
+    return ring_cameras(n, elevation_deg=35.0, radius=2.2 * extent + 4.0, target=(0.0, 0.5, 0.0), fov=45.0)
+
+
 def checker_texture(size=1024, tiles=16, seed=3, low=0.1, high=0.9):
     rng = np.random.default_rng(seed)
     cols = rng.uniform(low, high, (tiles, tiles, 3)).astype(np.float32)

@@ -33,3 +33,4 @@ def test_cornell_fixture_matches_fresh_oracle(oracle):
     assert np.mean(z > 3.0) < 0.02
     assert np.max(z) < 6.0
     assert np.all(np.isfinite(ref["mean"])) and np.all(ref["var"] >= 0)
+

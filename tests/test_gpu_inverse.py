@@ -38,3 +38,22 @@ def test_adam_on_adjoint_recovers_texture(fit):
     print("observed-texel RMSE", [f"{x:.4f}" for x in errs], flush=True)
     assert losses[-1] < 0.5 * losses[0]
     assert errs[-1] < 0.7 * errs[0]
+
+
+def test_heavy_case_workflow_small():
+    """The combined heavy workflow (forward, adjoint and denoiser on every view,
+    Adam on the device) at a reduced size: gradients reach all three textures
+    and the parameters move."""
+    h = I.HeavyCase(size=64, spp=2, views=2, tex=64, freq=6, instances=4, target_spp=8)
+    try:
+        th0 = [t.clone() for t in h.theta]
+        r1 = h.iteration(lr=0.05)
+        g = [float(t.abs().sum()) for t in h.grad]
+        r2 = h.iteration(lr=0.05)
+        print(r1, r2, g, flush=True)
+        assert all(np.isfinite([r1["loss"], r2["loss"]])) and r1["loss"] > 0
+        assert all(x > 0 for x in g)
+        assert all(float((a - b).abs().max()) > 0 for a, b in zip(h.theta, th0))
+        assert r1["forward_ms"] > 0 and r1["adjoint_ms"] > 0
+    finally:
+        h.close()
