@@ -73,6 +73,34 @@ def test_atrous_matches_oracle(spp):
     assert np.abs(out[..., :3] - col).mean() > 1e-3
 
 
+def test_display_sums_accumulate_per_sample():
+    """Two 1-spp frames: the display sums are display(x1) + display(x2) per
+    channel and lum(display(x1))^2 + lum(display(x2))^2, x1 and x2 read back
+    as the accumulation after each frame."""
+    W, H = 40, 30
+    sc = small_scene(W, H)
+    ctx = ohpt.Context(W, H)
+    try:
+        ctx.load_scene(sc)
+        D.with_guides(ctx, sc.settings)
+        ctx.set_render_seed(21)
+        ctx.reset()
+        ctx.render(1)
+        s1, _ = ctx.read_moments()
+        ctx.render(1)
+        s2, _ = ctx.read_moments()
+        disp = ctx.read_display_sums()
+    finally:
+        ctx.close()
+    x1 = s1[..., :3].astype(np.float64)
+    x2 = s2[..., :3].astype(np.float64) - x1
+    d1, d2 = atrous_ref.display(x1), atrous_ref.display(x2)
+    lum = lambda d: d @ atrous_ref.LUMA
+    assert np.all(s2[..., 3] == 2)
+    np.testing.assert_allclose(disp[..., :3], d1 + d2, atol=5e-5)
+    np.testing.assert_allclose(disp[..., 3], lum(d1) ** 2 + lum(d2) ** 2, atol=5e-5)
+
+
 def test_atrous_needs_aovs():
     sc = small_scene(32, 32)
     ctx = ohpt.Context(32, 32)
