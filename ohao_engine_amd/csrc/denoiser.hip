@@ -8,16 +8,17 @@
 // implicit GEMM on the gfx950 matrix cores, v_mfma_f32_16x16x32_bf16, FP32
 // accumulation; bias + ReLU are fused into the forward epilogue.
 //
-// Kernel shapes (wave64, 256-thread blocks = 4 waves):
-//   k_conv3x3<CC,MX> one block = (16*MX)x8 output pixels x 32 output channels.
-//                  Per input-channel chunk of CC channels the (16*MX+2)x10
+// Kernel shapes (wave64):
+//   k_conv3x3<CC,MX,RW> one block (RW/2 waves) = (16*MX)xRW output pixels x 32
+//                  output channels (RW = 16: 8 waves sharing one weight stage).
+//                  Per input-channel chunk of CC channels the (16*MX+2)x(RW+2)
 //                  input halo and the chunk's packed weights are staged in
 //                  LDS; each wave owns two rows of 16*MX pixels x 32 channels
 //                  (2*MX x 2 MFMA tiles, each weight fragment used 2*MX times)
 //                  and walks K = 9 taps x CC channels in steps of 32; the bf16
 //                  result is transposed through LDS so each output pixel's 32
 //                  channels are written as four 16-byte stores.
-//   k_wgrad<CC>    one block = (pixel-tile range, 32 output channels, CC
+//   k_wgrad<CC>    (256 threads) one block = (pixel-tile range, 32 output channels, CC
 //                  input channels); K = pixels, 32 per MFMA step; per-block
 //                  partial sums go to a buffer that a fixed-order reduction
 //                  sums (deterministic weight gradients).
@@ -76,8 +77,9 @@ struct ConvArgs {
     int B, H, W;
 };
 
-template <int CC, int MX>
-__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
+template <int CC, int MX, int RW>
+__global__ __launch_bounds__(32 * RW) void k_conv3x3(ConvArgs a) {
+    constexpr int NT = 32 * RW;              // threads: RW / 2 waves of two output rows each
     constexpr int KS = (9 * CC + 31) / 32;   // K steps of 32 per input chunk
     // LDS strides (bf16 elements), measured: a halo pixel padded by 8 and unpadded
     // weight rows beat the other 16-byte-aligned choices (0/16/24 and 40/48/56)
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     constexpr int VPP = CC / 8;              // 16-byte vectors per halo pixel
     constexpr int TX = 16 * MX;              // output pixels per block row
     constexpr int HX = TX + 2;               // halo row length
-    constexpr int HP = HX * 10;              // halo pixels
+    constexpr int HP = HX * (RW + 2);        // halo pixels
     __shared__ __align__(16) uint16_t xs[HP * XS];
     __shared__ __align__(16) uint16_t ws[KS * 32 * WS];
 
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     const int g = lane >> 4, r16 = lane & 15;
     const int ncout = a.np >> 5;
     const int b = blockIdx.z / ncout, nc = blockIdx.z - b * ncout;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * 8;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * RW;
     const int nchunk = (a.c0 + a.c1) / CC;
 
     // this wave: rows 2*wave, 2*wave+1; MX 16-pixel tiles per row; 2 tiles of 16 output channels
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
         __syncthreads();
         {
             // thread -> (halo pixel p, 16-byte vector v); p advances by 256 / VPP per step
-            constexpr int PSTEP = 256 / VPP, DY = PSTEP / HX, DX = PSTEP - DY * HX;
+            constexpr int PSTEP = NT / VPP, DY = PSTEP / HX, DX = PSTEP - DY * HX;
             const int v = tid % VPP;
             int p = tid / VPP;
             int hy = p / HX, hx = p - hy * HX;
@@ -132,7 +134,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
             }
         }
         const uint16_t* wsrc = a.wp + ((size_t)c * KS * a.np + (size_t)nc * 32) * 32;
-        for (int i = tid; i < KS * 128; i += 256) {
+        for (int i = tid; i < KS * 128; i += NT) {
             const int s = i >> 7, n = (i >> 2) & 31, v = i & 3;
             *reinterpret_cast<uint4*>(&ws[(s * 32 + n) * WS + v * 8]) =
                 *reinterpret_cast<const uint4*>(wsrc + ((size_t)s * a.np + n) * 32 + v * 8);
@@ -195,8 +197,9 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     // bf16 outputs: transpose the tile through LDS ([pixel][32 channels], padded) so each
     // pixel's 32 channels leave as four 16-byte stores
     constexpr int OS = 40;
-    static_assert(8 * TX * OS <= KS * 32 * WS, "output staging fits the weight buffer");
-    uint16_t* os = ws;
+    constexpr bool kStageInWeights = RW * TX * OS <= KS * 32 * WS;
+    static_assert(kStageInWeights || RW * TX * OS <= HP * XS, "output staging fits a shared buffer");
+    uint16_t* os = kStageInWeights ? ws : xs;
     __syncthreads();
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi) {
@@ -216,7 +219,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
         }
     }
     __syncthreads();
-    for (int i = tid; i < 8 * TX * 4; i += 256) {
+    for (int i = tid; i < RW * TX * 4; i += NT) {
         const int pl = i >> 2, v = i & 3;
         const int y = y0 + pl / TX, x = x0 + pl % TX;
         if (y >= a.H || x >= a.W) continue;
@@ -812,9 +815,13 @@ hipError_t ensure_capacity(ohpt_denoiser* d, int B, int Hp, int Wp) {
 // 16-pixel tiles per block row: 1 keeps LDS at 37 KB (4 blocks per CU); wider
 // tiles reuse each weight fragment more but fit fewer blocks and measured slower.
 constexpr int kConvMX = 1;
+constexpr int kConvRows = 16;
 
 void launch_conv(ohpt_denoiser* d, const Layer& L, const ConvArgs& a, int cc) {
-    dim3 grid(cdiv(a.W, 16 * kConvMX), cdiv(a.H, 8), (uint32_t)(a.B * (a.np / 32)));
+    // 16 output rows per block for 32-channel chunks (measured best of 8/16/24/32 at
+    // 1080p); the 16-channel input layer keeps 8, where the output staging would not fit
+    const int rows = cc == 16 ? 8 : kConvRows;
+    dim3 grid(cdiv(a.W, 16 * kConvMX), cdiv(a.H, rows), (uint32_t)(a.B * (a.np / 32)));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (d->profile) {
         if ((int)d->conv_ev.size() < d->conv_ev_used + 2) {
@@ -829,8 +836,8 @@ void launch_conv(ohpt_denoiser* d, const Layer& L, const ConvArgs& a, int cc) {
         d->conv_ev_used += 2;
         (void)hipEventRecord(e0, d->stream);
     }
-    if (cc == 16) k_conv3x3<16, kConvMX><<<grid, 256, 0, d->stream>>>(a);
-    else k_conv3x3<32, kConvMX><<<grid, 256, 0, d->stream>>>(a);
+    if (cc == 16) k_conv3x3<16, kConvMX, 8><<<grid, 256, 0, d->stream>>>(a);
+    else k_conv3x3<32, kConvMX, kConvRows><<<grid, 32 * kConvRows, 0, d->stream>>>(a);
     if (d->profile) (void)hipEventRecord(e1, d->stream);
     (void)L;
 }
