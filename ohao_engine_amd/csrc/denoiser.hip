@@ -418,15 +418,16 @@ __global__ __launch_bounds__(256) void k_relu_mask(uint16_t* __restrict__ g, con
 __global__ __launch_bounds__(256) void k_pool_fwd(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int B,
                                                   int H, int W, int C) {
     const int h = H >> 1, w = W >> 1, cv = C >> 3;
-    const size_t total = (size_t)B * h * w * cv;
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    // 32-bit index arithmetic: check_shape bounds B*Hp*Wp by 2^28, so B*h*w*cv < 2^32
+    const uint32_t total = (uint32_t)B * h * w * cv;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= total) return;
-    const int v = (int)(i % cv);
-    size_t r = i / cv;
-    const int x = (int)(r % w);
-    r /= w;
-    const int y = (int)(r % h);
-    const int b = (int)(r / h);
+    const int v = (int)(i % (uint32_t)cv);
+    uint32_t r = i / (uint32_t)cv;
+    const int x = (int)(r % (uint32_t)w);
+    r /= (uint32_t)w;
+    const int y = (int)(r % (uint32_t)h);
+    const int b = (int)(r / (uint32_t)h);
     uint4 q[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -453,15 +454,16 @@ __global__ __launch_bounds__(256) void k_pool_bwd(const uint16_t* __restrict__ i
                                                   uint16_t* __restrict__ gin, int accumulate, int B, int H, int W,
                                                   int C) {
     const int h = H >> 1, w = W >> 1, cv = C >> 3;
-    const size_t total = (size_t)B * h * w * cv;
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    // 32-bit index arithmetic: check_shape bounds B*Hp*Wp by 2^28, so B*h*w*cv < 2^32
+    const uint32_t total = (uint32_t)B * h * w * cv;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= total) return;
-    const int v = (int)(i % cv);
-    size_t r = i / cv;
-    const int x = (int)(r % w);
-    r /= w;
-    const int y = (int)(r % h);
-    const int b = (int)(r / h);
+    const int v = (int)(i % (uint32_t)cv);
+    uint32_t r = i / (uint32_t)cv;
+    const int x = (int)(r % (uint32_t)w);
+    r /= (uint32_t)w;
+    const int y = (int)(r % (uint32_t)h);
+    const int b = (int)(r / (uint32_t)h);
     uint4 q[4], gi[4];
     size_t off[4];
 #pragma unroll
@@ -499,15 +501,16 @@ __device__ inline void up_taps(int X, int n, int& i0, int& i1, float& l1) {
 __global__ __launch_bounds__(256) void k_up_fwd(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int B,
                                                 int h, int w, int C) {
     const int H = 2 * h, W = 2 * w, cv = C >> 3;
-    const size_t total = (size_t)B * H * W * cv;
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    // 32-bit index arithmetic: check_shape bounds B*Hp*Wp by 2^28, so B*H*W*cv < 2^32
+    const uint32_t total = (uint32_t)B * H * W * cv;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= total) return;
-    const int v = (int)(i % cv);
-    size_t r = i / cv;
-    const int X = (int)(r % W);
-    r /= W;
-    const int Y = (int)(r % H);
-    const int b = (int)(r / H);
+    const int v = (int)(i % (uint32_t)cv);
+    uint32_t r = i / (uint32_t)cv;
+    const int X = (int)(r % (uint32_t)W);
+    r /= (uint32_t)W;
+    const int Y = (int)(r % (uint32_t)H);
+    const int b = (int)(r / (uint32_t)H);
     int y0, y1, x0, x1;
     float ly, lx;
     up_taps(Y, h, y0, y1, ly);
@@ -545,15 +548,16 @@ __device__ inline float up_weight(int X, int n, int i) {
 __global__ __launch_bounds__(256) void k_up_bwd(const uint16_t* __restrict__ gout, uint16_t* __restrict__ gin, int B,
                                                 int h, int w, int C) {
     const int H = 2 * h, W = 2 * w, cv = C >> 3;
-    const size_t total = (size_t)B * h * w * cv;
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    // 32-bit index arithmetic: check_shape bounds B*Hp*Wp by 2^28, so B*h*w*cv < 2^32
+    const uint32_t total = (uint32_t)B * h * w * cv;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= total) return;
-    const int v = (int)(i % cv);
-    size_t r = i / cv;
-    const int x = (int)(r % w);
-    r /= w;
-    const int y = (int)(r % h);
-    const int b = (int)(r / h);
+    const int v = (int)(i % (uint32_t)cv);
+    uint32_t r = i / (uint32_t)cv;
+    const int x = (int)(r % (uint32_t)w);
+    r /= (uint32_t)w;
+    const int y = (int)(r % (uint32_t)h);
+    const int b = (int)(r / (uint32_t)h);
     float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int Y = max(0, 2 * y - 2); Y <= min(H - 1, 2 * y + 2); ++Y) {
         const float wy = up_weight(Y, h, y);
@@ -579,12 +583,12 @@ __global__ __launch_bounds__(256) void k_up_bwd(const uint16_t* __restrict__ gou
 __global__ __launch_bounds__(256) void k_features(const float4* __restrict__ rad, const float4* __restrict__ alb,
                                                   const float4* __restrict__ nrm, const float* __restrict__ dep,
                                                   uint16_t* __restrict__ x0, int B, int H, int W, int Hp, int Wp) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t total = (size_t)B * Hp * Wp;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;   // B*Hp*Wp <= 2^28 (check_shape)
+    const uint32_t total = (uint32_t)B * Hp * Wp;
     if (i >= total) return;
-    const int x = (int)(i % Wp);
-    const int y = (int)((i / Wp) % Hp);
-    const int b = (int)(i / ((size_t)Wp * Hp));
+    const int x = (int)(i % (uint32_t)Wp);
+    const int y = (int)((i / (uint32_t)Wp) % (uint32_t)Hp);
+    const int b = (int)(i / ((uint32_t)Wp * Hp));
     const size_t s = ((size_t)b * H + min(y, H - 1)) * W + min(x, W - 1);
     float f[16];
     const float4 r = rad[s];
@@ -602,8 +606,8 @@ __global__ __launch_bounds__(256) void k_features(const float4* __restrict__ rad
     uint16_t* oe = reinterpret_cast<uint16_t*>(o);
 #pragma unroll
     for (int k = 0; k < 16; ++k) oe[k] = f2bf(f[k]);
-    reinterpret_cast<uint4*>(x0)[i * 2] = o[0];
-    reinterpret_cast<uint4*>(x0)[i * 2 + 1] = o[1];
+    reinterpret_cast<uint4*>(x0)[(size_t)i * 2] = o[0];
+    reinterpret_cast<uint4*>(x0)[(size_t)i * 2 + 1] = o[1];
 }
 
 // Tonemap of the training loss: T(x) = (ACES(0.5 x) + 1e-3)^(1/2.2), ACES
@@ -639,15 +643,15 @@ struct KpnArgs {
 // 5x5 window), and for training the L1 tonemapped loss with its gradient.
 __global__ __launch_bounds__(256) void k_kpn(KpnArgs a) {
     __shared__ float red[256];
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t total = (size_t)a.B * a.Hp * a.Wp;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;   // B*Hp*Wp <= 2^28 (check_shape)
+    const uint32_t total = (uint32_t)a.B * a.Hp * a.Wp;
     float lsum = 0.0f;
     if (i < total) {
-        const int x = (int)(i % a.Wp);
-        const int y = (int)((i / a.Wp) % a.Hp);
-        const int b = (int)(i / ((size_t)a.Wp * a.Hp));
+        const int x = (int)(i % (uint32_t)a.Wp);
+        const int y = (int)((i / (uint32_t)a.Wp) % (uint32_t)a.Hp);
+        const int b = (int)(i / ((uint32_t)a.Wp * a.Hp));
         if (y < a.H && x < a.W) {
-            const float* lg = a.logits + i * 32;
+            const float* lg = a.logits + (size_t)i * 32;
             float s[25];
             float mx = lg[0];
 #pragma unroll
@@ -687,12 +691,12 @@ __global__ __launch_bounds__(256) void k_kpn(KpnArgs a) {
                     for (int k = 25; k < 32; ++k) dl[k] = 0;
 #pragma unroll
                     for (int v = 0; v < 4; ++v)
-                        reinterpret_cast<uint4*>(a.dlogits + i * 32)[v] = reinterpret_cast<const uint4*>(dl)[v];
+                        reinterpret_cast<uint4*>(a.dlogits + (size_t)i * 32)[v] = reinterpret_cast<const uint4*>(dl)[v];
                 }
             }
         } else if (a.dlogits) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) reinterpret_cast<uint4*>(a.dlogits + i * 32)[v] = make_uint4(0u, 0u, 0u, 0u);
+            for (int v = 0; v < 4; ++v) reinterpret_cast<uint4*>(a.dlogits + (size_t)i * 32)[v] = make_uint4(0u, 0u, 0u, 0u);
         }
     }
     if (a.loss_part) {
