@@ -1,0 +1,77 @@
+"""Error behaviour of the C-ABI (status codes + a message for
+ohpt_last_error, the counterpart of the reference's bool + std::cerr
+convention, path_tracer.cpp:92-120): calls out of order, bad arguments, and
+the degenerate empty scene, which renders the environment."""
+import re
+
+import numpy as np
+import pytest
+
+from ohao_engine_amd import ohpt, scenes as S
+
+pytestmark = pytest.mark.gpu
+
+INVALID_ARG, NO_SCENE, STATE = -1, -3, -6
+
+
+def code(fn):
+    with pytest.raises(ohpt.OhptError) as e:
+        fn()
+    msg = str(e.value)
+    m = re.search(r"failed \((-?\d+)\): (.+)", msg)
+    assert m and m.group(2).strip(), msg       # a non-empty message from ohpt_last_error
+    return int(m.group(1))
+
+
+def test_calls_out_of_order():
+    ctx = ohpt.Context(16, 16)
+    try:
+        rays = np.zeros((4, 8), np.float32)
+        rays[:, 6] = 1.0
+        rays[:, 7] = np.inf
+        assert code(lambda: ctx.render(1)) == NO_SCENE
+        assert code(lambda: ctx.trace_rays(rays)) == NO_SCENE
+        assert code(lambda: ctx.refit()) == NO_SCENE
+        assert code(lambda: ctx.adjoint(np.zeros((16, 16, 4), np.float32), 1)) == NO_SCENE
+        tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+        assert code(lambda: ctx.add_mesh(tri, np.arange(3, dtype=np.uint32))) == STATE
+        assert code(lambda: ctx.commit_scene()) == STATE
+        assert code(lambda: ctx.env_sample(np.full((2, 2), 0.5, np.float32))) == STATE
+    finally:
+        ctx.close()
+
+
+def test_bad_arguments():
+    sc = S.cornell_box(width=16, height=16)
+    ctx = ohpt.Context(16, 16)
+    try:
+        ctx.load_scene(sc)
+        bad = S.Settings(max_bounces=61)
+        assert code(lambda: ctx.set_settings(bad)) == INVALID_ARG
+        mats = np.array([S.material((0.5, 0.5, 0.5), albedo_tex=3)], S.MATERIAL_DTYPE)
+        assert code(lambda: ctx.set_materials(mats)) == INVALID_ARG
+        assert code(lambda: ctx.set_mesh_transform(7, np.eye(4))) == INVALID_ARG
+        # the scene still renders after the refused calls
+        ctx.render(1)
+        assert np.all(ctx.read_accum()[..., 3] == 1)
+    finally:
+        ctx.close()
+
+
+def test_empty_scene_renders_the_environment():
+    sc = S.Scene(pos=np.zeros((0, 9), np.float32), mat=np.zeros(0, np.uint32),
+                 materials=np.array([S.lambert((0.5, 0.5, 0.5))]), env=S.constant_env(0.5),
+                 camera=S.Camera.look_at((0, 0, 3), (0, 0, 0), 40.0), width=24, height=16)
+    ctx = ohpt.Context(24, 16)
+    try:
+        ctx.load_scene(sc)
+        ctx.render(4)
+        img = ctx.read_accum()
+        assert np.all(img[..., :3] == np.float32(0.5)) and np.all(img[..., 3] == 4)
+        rays = np.zeros((8, 8), np.float32)
+        rays[:, 4:7] = [0.0, 0.0, -1.0]
+        rays[:, 7] = np.inf
+        hits = ctx.trace_rays(rays)
+        assert np.all(hits[:, 1].view(np.uint32) == 0xFFFFFFFF)
+    finally:
+        ctx.close()
