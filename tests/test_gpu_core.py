@@ -204,12 +204,6 @@ def test_env_cdf_matches_oracle(oracle):
     k = np.unravel_index(np.argmax(dc), dc.shape)
     log(f"env cdf: marginal max diff {dm.max():.3e}, conditional max diff {dc.max():.3e} at {k} "
         f"(gpu {gc[k]:.9g} oracle {cc[k]:.9g}), exact {np.mean(gc == cc):.4f}, integral {gi} vs {ci}")
-    try:
-        import os
-        os.makedirs("gpurun_out", exist_ok=True)
-        np.savez_compressed("gpurun_out/env_cdf_debug.npz", gm=gm, gc=gc, cm=cm, cc=cc)
-    except OSError:
-        pass
     # Rows are summed in the reference's order; sinf may differ in the last
     # place, which rescales a row's partial sums by ~1 ulp before normalising.
     assert dm.max() < 1e-5 and dc.max() < 1e-5
