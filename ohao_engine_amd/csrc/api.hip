@@ -46,7 +46,6 @@ struct TexStore {
     size_t packed_floats() const { return (size_t)w * h * (c >= 2 ? 4u : 1u); }
 };
 
-inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 }  // namespace
 
@@ -1315,7 +1314,7 @@ int ohpt_denoise_atrous(ohpt_context* c, float* out, int on_device) {
     HIPCHK(c->dn_guides.ensure(svgf_scratch_bytes(c->W, c->H)));
     const uint32_t n = c->W * c->H;
     return produce(c, out, (size_t)n * 16, on_device, [&](void* dst) {
-        svgf_atrous(c->d_accum.as<float4>(), c->d_disp.as<float4>(), c->d_aov_albedo.as<float4>(),
+        (void)svgf_atrous(c->d_accum.as<float4>(), c->d_disp.as<float4>(), c->d_aov_albedo.as<float4>(),
                     c->d_aov_normal.as<float4>(), c->W, c->H, reinterpret_cast<float4*>(dst), c->dn_guides.p,
                     c->stream);
     });

@@ -529,8 +529,9 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, uint32_t
         std::swap(va, vb);
     }
     if (ka != keys) {
-        hipMemcpyAsync(keys, ka, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
-        hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+        hipError_t e = hipMemcpyAsync(keys, ka, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
     }
     return hipGetLastError();
 }
@@ -625,10 +626,6 @@ hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out, 
     uint32_t* digit = reinterpret_cast<uint32_t*>(sc + L.digit);
     uint32_t* node_parent = reinterpret_cast<uint32_t*>(sc + L.node_parent);
     uint32_t* leaf_parent = reinterpret_cast<uint32_t*>(sc + L.leaf_parent);
-    uint32_t* node_depth = reinterpret_cast<uint32_t*>(sc + L.node_depth);
-    uint32_t* level_count = reinterpret_cast<uint32_t*>(sc + L.levels);
-    uint32_t* level_cursor = level_count + kMaxLevels;
-    uint32_t* level_nodes = reinterpret_cast<uint32_t*>(sc + L.level_nodes);
 
     const uint32_t nparts = std::min<uint32_t>(1024u, div_up(n, 256));
     k_centroid_bounds<<<nparts, 256, 0, s>>>(b.tri_pos, n, partial);
@@ -647,7 +644,6 @@ hipError_t lbvh_build(const LbvhBuffers& b, hipStream_t s, uint32_t* depth_out, 
     k_gather<<<div_up(n, 256), 256, 0, s>>>(b.tri_pos, b.tri_nrm, b.tri_uv, b.tri_mat, va, n, b.tris, b.attrs,
                                              b.prim, b.slot_of);
     uint32_t* nodes_u = reinterpret_cast<uint32_t*>(b.nodes);
-    const uint32_t ni = n > 1 ? n - 1 : 1;
     if (n == 1) {
         k_single_leaf<<<1, 1, 0, s>>>(nodes_u, node_parent, leaf_parent);
     } else {

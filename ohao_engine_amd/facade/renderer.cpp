@@ -133,7 +133,7 @@ void Renderer::setRTRenderSettings(const RTRenderSettings& s) {
     st.max_bounces = s.maxBounces;
     st.rr_start = s.rrStart;
     st.sampler = s.sobol ? OHPT_SAMPLER_SOBOL_OWEN : OHPT_SAMPLER_PCG32;
-    st.flags = OHPT_FLAG_AOVS | (s.fireflyClamp ? OHPT_FLAG_FIREFLY_CLAMP : 0u);
+    st.flags = (uint32_t)OHPT_FLAG_AOVS | (s.fireflyClamp ? (uint32_t)OHPT_FLAG_FIREFLY_CLAMP : 0u);
     st.firefly_clamp = s.fireflyClampValue;
     if (ohpt_set_settings(ctx_, &st) != OHPT_OK) fail("ohpt_set_settings");
 }
@@ -486,7 +486,7 @@ bool Renderer::adjoint(const std::vector<float>& dl_dimage, uint32_t spp, bool d
     if (!ensure_scene()) return false;
     const ohpt_camera cam = camera_.toOhpt();
     if (ohpt_set_camera(ctx_, &cam) != OHPT_OK) return fail("ohpt_set_camera");
-    if (ohpt_adjoint(ctx_, dl_dimage.data(), 0, spp, deterministic ? OHPT_ADJOINT_DETERMINISTIC : 0u) != OHPT_OK)
+    if (ohpt_adjoint(ctx_, dl_dimage.data(), 0, spp, deterministic ? (uint32_t)OHPT_ADJOINT_DETERMINISTIC : 0u) != OHPT_OK)
         return fail("ohpt_adjoint");
     return true;
 }
