@@ -144,3 +144,46 @@ def test_atrous_denoise_mode_through_facade(tmp_path):
 def test_inverse_fit_driver_selftest(tmp_path):
     out = run([BIN / "inverse_fit", "--selftest"], tmp_path, timeout=300)
     assert "Done:" in out
+
+
+def write_rgbe(path, w, h, m, e, rle):
+    """Constant Radiance .hdr: every texel mantissa m, exponent e, as flat RGBE
+    pixels or as run-length encoded scanlines (one component at a time)."""
+    rows = []
+    for _ in range(h):
+        if not rle:
+            rows.append(bytes([m, m, m, e]) * w)
+            continue
+        row = bytearray([2, 2, w >> 8, w & 255])
+        for v in (m, m, m, e):
+            left = w
+            while left:
+                n = min(left, 127)
+                row += bytes([128 + n, v])
+                left -= n
+        rows.append(bytes(row))
+    head = f"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y {h} +X {w}\n".encode()
+    Path(path).write_bytes(head + b"".join(rows))
+
+
+def test_hdr_environment_map_through_facade(tmp_path):
+    """setEnvironmentMap reads a Radiance RGBE file (flat and run-length encoded
+    scanlines decode alike), and camera rays that escape see its radiance: a
+    constant 0.75 environment shows as display(0.75) in the sky pixels."""
+    frames = []
+    for rle in (False, True):
+        env = tmp_path / f"env_{int(rle)}.hdr"
+        write_rgbe(env, 64, 32, 192, 128, rle)   # 192 * 2^(128-136) = 0.75
+        out_dir = tmp_path / f"env_{int(rle)}"
+        run([BIN / "turntable", "sphere", "env", 4, 1, "--size=64x48", f"--env={env}", f"--out={out_dir}"], tmp_path)
+        fr = sorted(out_dir.glob("env_*.png"))
+        assert len(fr) == 1
+        frames.append(read_png_rgba8(fr[0]).astype(np.int32))
+    assert np.array_equal(frames[0], frames[1])
+    x = np.float32(0.75 * 0.5)
+    aces = min(1.0, max(0.0, (x * (2.51 * x + 0.03)) / (x * (2.43 * x + 0.59) + 0.14)))
+    want = int(aces ** (1 / 2.2) * 255 + 0.5)
+    top = frames[0][0, :, :3]
+    sky = (np.abs(top - want) <= 1).all(-1).mean()
+    print(f"expected sky value {want}; top row {top[:, 0].tolist()}; fraction at it {sky:.2f}")
+    assert sky > 0.25
